@@ -1,0 +1,190 @@
+"""ctypes binding of libffip_hip.so (include/ffip_hip.h).
+
+The library is the product path: there is no CPU fallback.  Loading fails loudly
+when the shared object has not been built, and every failing call raises
+``FfipError`` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libffip_hip.so")
+
+_lib = None
+
+
+class FfipError(RuntimeError):
+    pass
+
+
+_SIGS = {
+    "ffip_last_error": ([], ctypes.c_char_p),
+    "ffip_sim_create": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+    "ffip_sim_set_dataset": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_size_t], ctypes.c_int),
+    "ffip_sim_add_probe": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p],
+                           ctypes.c_int),
+    "ffip_sim_finalize_setup": ([ctypes.c_void_p], ctypes.c_int),
+    "ffip_sim_run": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "ffip_sim_step": ([ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "ffip_sim_time_step": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "ffip_sim_dft_count": ([ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "ffip_sim_dft_dims": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sim_dft_group": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t], ctypes.c_int),
+    "ffip_sim_get_dft": ([ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 6, ctypes.c_int),
+    "ffip_sim_set_trace_capacity": ([ctypes.c_void_p, ctypes.c_int64], ctypes.c_int),
+    "ffip_sim_get_trace": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                            ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "ffip_sim_probe": ([ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p,
+                        ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    "ffip_sim_component_dims": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sim_get_component": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sim_geom_info_count": ([ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)], ctypes.c_int),
+    "ffip_sim_get_geom_info": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+                               ctypes.c_int),
+    "ffip_sim_step_timed": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sim_info": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sim_destroy": ([ctypes.c_void_p], None),
+}
+
+
+def lib():
+    """Load libffip_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FfipError("libffip_hip.so is not built (run `python -m ffip_amd.build`); "
+                            "ffip_amd has no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().ffip_last_error()
+        raise FfipError("ffip error %d: %s" % (rc, msg.decode() if msg else "?"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class LibSim:
+    """One simulation handle of the C ABI."""
+
+    def __init__(self, config, datasets: Optional[Dict[str, np.ndarray]] = None,
+                 probes: Sequence = (), trace_steps: int = 0):
+        L = lib()
+        text = config if isinstance(config, str) else json.dumps(config)
+        h = ctypes.c_void_p()
+        _check(L.ffip_sim_create(text.encode(), ctypes.byref(h)))
+        self._h = h
+        for name, arr in (datasets or {}).items():
+            a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel())
+            _check(L.ffip_sim_set_dataset(self._h, name.encode(), _ptr(a), a.size))
+        for pos, comp in probes:
+            _check(L.ffip_sim_add_probe(self._h, float(pos[0]), float(pos[1]), float(pos[2]), comp.encode()))
+        if trace_steps:
+            _check(L.ffip_sim_set_trace_capacity(self._h, int(trace_steps)))
+        _check(L.ffip_sim_finalize_setup(self._h))
+        self.nprobes = len(probes)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ffip_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- stepping
+    def run(self, stop) -> int:
+        n = ctypes.c_int64()
+        text = stop if isinstance(stop, str) else json.dumps(stop)
+        _check(lib().ffip_sim_run(self._h, text.encode(), ctypes.byref(n)))
+        return int(n.value)
+
+    def step(self, n: int):
+        _check(lib().ffip_sim_step(self._h, int(n)))
+
+    @property
+    def time_step(self) -> int:
+        n = ctypes.c_int64()
+        _check(lib().ffip_sim_time_step(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def step_timed(self, n: int, per_kernel: bool = True):
+        out = np.zeros(3)
+        _check(lib().ffip_sim_step_timed(self._h, int(n), 1 if per_kernel else 0, _ptr(out)))
+        return dict(total_ms=out[0], e_ms=out[1], h_ms=out[2])
+
+    def info(self):
+        out = np.zeros(6)
+        _check(lib().ffip_sim_info(self._h, _ptr(out)))
+        return dict(cells=(out[0], out[1], out[2]), device_bytes=out[3], dt=out[4], dx=out[5])
+
+    # -- results
+    def dft_count(self) -> int:
+        n = ctypes.c_int()
+        _check(lib().ffip_sim_dft_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def dft(self, k: int):
+        dims = np.zeros(4, dtype=np.int64)
+        _check(lib().ffip_sim_dft_dims(self._h, int(k), _ptr(dims)))
+        nf, nz, ny, nx = (int(v) for v in dims)
+        re = np.zeros((nf, nz, ny, nx))
+        im = np.zeros_like(re)
+        x, y, z, f = np.zeros(nx), np.zeros(ny), np.zeros(nz), np.zeros(nf)
+        _check(lib().ffip_sim_get_dft(self._h, int(k), _ptr(re), _ptr(im), _ptr(x), _ptr(y), _ptr(z), _ptr(f)))
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().ffip_sim_dft_group(self._h, int(k), buf, 256))
+        return dict(real=re, imag=im, x=x, y=y, z=z, f=f, group=buf.value.decode())
+
+    def trace(self):
+        n = ctypes.c_int64()
+        p = ctypes.c_int()
+        _check(lib().ffip_sim_get_trace(self._h, None, 0, ctypes.byref(n), ctypes.byref(p)))
+        out = np.zeros((int(n.value), max(1, p.value)))
+        if n.value and p.value:
+            _check(lib().ffip_sim_get_trace(self._h, _ptr(out), int(n.value), None, None))
+        return out[:, :p.value]
+
+    def probe(self, pos, comp: str) -> float:
+        v = ctypes.c_double()
+        _check(lib().ffip_sim_probe(self._h, float(pos[0]), float(pos[1]), float(pos[2]), comp.encode(),
+                                    ctypes.byref(v)))
+        return v.value
+
+    def component(self, comp: str) -> np.ndarray:
+        dims = np.zeros(3, dtype=np.int64)
+        _check(lib().ffip_sim_component_dims(self._h, comp.encode(), _ptr(dims)))
+        out = np.zeros(tuple(int(d) for d in dims))
+        _check(lib().ffip_sim_get_component(self._h, comp.encode(), _ptr(out)))
+        return out
+
+    def geom_info(self, k: int):
+        n = ctypes.c_int64()
+        _check(lib().ffip_sim_geom_info_count(self._h, int(k), ctypes.byref(n)))
+        x, y, z = np.zeros(n.value), np.zeros(n.value), np.zeros(n.value)
+        _check(lib().ffip_sim_get_geom_info(self._h, int(k), _ptr(x), _ptr(y), _ptr(z)))
+        return x, y, z

@@ -1,0 +1,1163 @@
+// Fused FDTD half-step kernels for gfx950 and the device engine around them.
+//
+// One launch per half step (k_half<IS_E>) does, for every real point of the three
+// components it updates, what the reference does in four passes:
+//   curl + CPML psi      (PML_Stepping::step / Curl_Stepping::step, src/fields.cpp:89-148)
+//   ADE poles + update   (Medium_Stepping::step, src/structure.cpp:56-85)
+//   DFT accumulation     (DFT_Unit::update, src/dft.cpp:29-62)
+// Point sources (Gaussian_Dipoles_Stepping::step, src/fields.cpp:17-33) touch a
+// few points only and run in a small follow-up kernel that also patches the DFT
+// sums of the points it changes; the ghost fills of Period/PEC/PMC boundaries
+// (Fields::sync_boundary, src/fields.cpp:327-482) are a third small kernel.
+//
+// The update is written in "E form": E^{n+1} = E^n + (dt*curl - (sum P^{n+1} -
+// sum P^n)) / eps_inf.  Algebraically this is the reference's D form
+// (E = (D - sum P)/eps_inf with D += dt*curl, src/structure.cpp:83) since
+// D^n = eps_inf E^n + sum P^n; it needs no D array.  The polarization difference is
+// formed from the stored (rounded) P values so that D stays consistent when
+// fields are stored in FP32.  All arithmetic is FP64; T is the storage type.
+//
+// Tiling: a 64x4 workgroup owns an x-y tile and marches a z segment, carrying the
+// two z-neighbour components in registers; x/y neighbours come from L1/L2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "engine.hpp"
+
+namespace ffip_amd {
+
+#define HIP_OK(expr)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr); \
+    } while (0)
+
+static constexpr int BX = 64;
+static constexpr int BY = 4;
+static constexpr int LZ = 32;
+static constexpr int MAXU = 16;
+static constexpr int MAXS = 8;  // polarization slots per point
+static constexpr int PH_CHUNK = 1024;
+
+struct DftDev {
+    int comp;     // 0..2 within the half
+    int box[3][2];
+    int dims[3];
+    int nf, mf0, is_db;
+    int64_t npts;
+    double* re;
+    double* im;
+};
+
+struct PmlDev {
+    const int* smap;
+    const double* b;
+    const double* c;
+    const double* ik;
+    void* psi1;  // term-1 auxiliary of component (a+2)%3
+    void* psi2;  // term-2 auxiliary of component (a+1)%3
+    int W;
+};
+
+template <typename T>
+struct HalfArgs {
+    T* u[3];
+    const T* s[3];
+    int64_t pitch, plane;
+    int n[3];
+    int lo[3][3], hi[3][3];
+    int dlo[3], dhi[3];
+    double inv_dx, dt;
+    // materials
+    double u_inv, u_eps;
+    const uint8_t* ids[3];
+    const double* m_inv;
+    const double* m_eps;
+    const int* m_np;
+    const int* m_p0;
+    const double* p_c1;
+    const double* p_c2;
+    const double* p_sc3;
+    int d_present[3];
+    int d_box[3][3][2];
+    int d_np[3];
+    const double* d_inv[3];
+    const double* d_eps[3];
+    const double* d_amp[3];
+    const double* d_c[3];  // [np][3] = (c1, c2, c3)
+    // polarization state: slot s, component c at (s*3 + c) * pstride
+    T* Pcur;
+    T* Pprev;
+    int64_t pstride;
+    PmlDev pml[3];
+    int nunits;
+    DftDev units[MAXU];
+    const double* ph;  // this half step's phasor row [mf][2] = (cos wt, -sin wt)
+};
+
+// Gaussian1 pulse (src/utility.cpp:260-264)
+__device__ __forceinline__ double gaussian1(double t, double width)
+{
+    double arg = t / width;
+    return arg * exp(-arg * arg / 2);
+}
+
+// psi index of a point in the slab of axis a (slab position s)
+__device__ __forceinline__ int64_t psi_index(int a, int s, int i, int j, int k, const int* n, int W)
+{
+    if (a == 0)
+        return int64_t(s) + int64_t(W) * (j + int64_t(n[1]) * k);
+    if (a == 1)
+        return int64_t(i) + int64_t(n[0]) * (s + int64_t(W) * k);
+    return int64_t(i) + int64_t(n[0]) * (j + int64_t(n[1]) * s);
+}
+
+template <typename T, int C>
+__device__ __forceinline__ double cpml(const HalfArgs<T>& A, int a, bool first, int i, int j, int k, double d)
+{
+    const PmlDev& P = A.pml[a];
+    if (P.W == 0)
+        return d;
+    const int coord = a == 0 ? i : (a == 1 ? j : k);
+    const int s = P.smap[coord];
+    if (s < 0)
+        return d;
+    T* psi = static_cast<T*>(first ? P.psi1 : P.psi2);
+    const int64_t q = psi_index(a, s, i, j, k, A.n, P.W);
+    const double ps = P.b[coord] * double(psi[q]) + P.c[coord] * d;
+    psi[q] = T(ps);
+    return ps + d * P.ik[coord];
+}
+
+template <typename T, bool MAT, int C>
+__device__ __forceinline__ void update_point(const HalfArgs<T>& A, int i, int j, int k, int64_t idx, double d1,
+                                             double d2)
+{
+    constexpr int a1 = (C + 1) % 3;
+    constexpr int a2 = (C + 2) % 3;
+    const double t1 = cpml<T, C>(A, a1, true, i, j, k, d1);
+    const double t2 = cpml<T, C>(A, a2, false, i, j, k, d2);
+    const double acc = t1 - t2;  // increment of D/dt (B/dt)
+    T* __restrict__ u = A.u[C];
+    const double e_old = double(u[idx]);
+    double e_new, eps, sumP = 0;
+    if (!MAT) {
+        e_new = e_old + A.dt * acc * A.u_inv;
+        eps = A.u_eps;
+    } else {
+        const int id = A.ids[C][idx];
+        double inv, dP = 0;
+        if (id != 255) {
+            inv = A.m_inv[id];
+            eps = A.m_eps[id];
+            const int np = A.m_np[id];
+            const int p0 = A.m_p0[id];
+            for (int s = 0; s < np; ++s) {
+                const int64_t o = (int64_t(s) * 3 + C) * A.pstride + idx;
+                const double pc = double(A.Pcur[o]);
+                const double pp = double(A.Pprev[o]);
+                const double pn = A.p_c1[p0 + s] * pc + A.p_c2[p0 + s] * pp + A.p_sc3[p0 + s] * e_old;
+                const T pt = T(pn);
+                A.Pprev[o] = pt;
+                dP += double(pt) - pc;
+                sumP += double(pt);
+            }
+        } else {
+            const int b00 = A.d_box[C][0][0], b10 = A.d_box[C][1][0], b20 = A.d_box[C][2][0];
+            const int64_t d0 = A.d_box[C][0][1] - b00 + 1, d1n = A.d_box[C][1][1] - b10 + 1;
+            const int64_t q = (int64_t(k - b20) * d1n + (j - b10)) * d0 + (i - b00);
+            inv = A.d_inv[C][q];
+            eps = A.d_eps[C][q];
+            const int np = A.d_np[C];
+            for (int s = 0; s < np; ++s) {
+                const int64_t o = (int64_t(s) * 3 + C) * A.pstride + idx;
+                const double pc = double(A.Pcur[o]);
+                const double pp = double(A.Pprev[o]);
+                const double amp = A.d_amp[C][q * np + s];
+                const double* cc = A.d_c[C] + 3 * s;
+                const double pn = cc[0] * pc + cc[1] * pp + amp * cc[2] * e_old;
+                const T pt = T(pn);
+                A.Pprev[o] = pt;
+                dP += double(pt) - pc;
+                sumP += double(pt);
+            }
+        }
+        e_new = e_old + (A.dt * acc - dP) * inv;
+    }
+    const T stored = T(e_new);
+    u[idx] = stored;
+    // DFT monitors (k is uniform across the workgroup)
+    for (int m = 0; m < A.nunits; ++m) {
+        const DftDev& U = A.units[m];
+        if (U.comp != C || k < U.box[2][0] || k > U.box[2][1])
+            continue;
+        if (i < U.box[0][0] || i > U.box[0][1] || j < U.box[1][0] || j > U.box[1][1])
+            continue;
+        const double v = U.is_db ? eps * double(stored) + sumP : double(stored);
+        const int64_t off = (i - U.box[0][0]) + int64_t(U.dims[0]) * ((j - U.box[1][0]) + int64_t(U.dims[1]) * (k - U.box[2][0]));
+        const double* ph = A.ph + 2 * U.mf0;
+        for (int f = 0; f < U.nf; ++f) {
+            const int64_t o = int64_t(f) * U.npts + off;
+            U.im[o] += ph[2 * f + 1] * v;
+            U.re[o] += ph[2 * f] * v;
+        }
+    }
+}
+
+template <typename T, bool IS_E, bool MAT>
+__global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
+{
+    const int i = A.dlo[0] + int(blockIdx.x) * BX + int(threadIdx.x);
+    const int j = A.dlo[1] + int(blockIdx.y) * BY + int(threadIdx.y);
+    const int kb = A.dlo[2] + int(blockIdx.z) * LZ;
+    const int ke = min(kb + LZ - 1, A.dhi[2]);
+    if (i > A.dhi[0] || j > A.dhi[1])
+        return;
+    const int64_t sy = A.pitch, sz = A.plane;
+    const double inv_dx = A.inv_dx;
+    const T* __restrict__ s0 = A.s[0];
+    const T* __restrict__ s1 = A.s[1];
+    const T* __restrict__ s2 = A.s[2];
+    const bool in0 = i >= A.lo[0][0] && i <= A.hi[0][0] && j >= A.lo[0][1] && j <= A.hi[0][1];
+    const bool in1 = i >= A.lo[1][0] && i <= A.hi[1][0] && j >= A.lo[1][1] && j <= A.hi[1][1];
+    const bool in2 = i >= A.lo[2][0] && i <= A.hi[2][0] && j >= A.lo[2][1] && j <= A.hi[2][1];
+    int64_t idx = i + sy * j + sz * int64_t(kb);
+    // z carry: E needs s0,s1 at k and k+1; H needs them at k-1 and k
+    T cx, cy;
+    if (IS_E) {
+        cx = s0[idx];
+        cy = s1[idx];
+    } else {
+        cx = s0[idx - sz];
+        cy = s1[idx - sz];
+    }
+    for (int k = kb; k <= ke; ++k, idx += sz) {
+        T nx, ny;
+        if (IS_E) {
+            nx = s0[idx + sz];
+            ny = s1[idx + sz];
+        } else {
+            nx = s0[idx];
+            ny = s1[idx];
+        }
+        const T c2 = s2[idx];
+        // component x: d/dy of s2, d/dz of s1
+        if (in0 && k >= A.lo[0][2] && k <= A.hi[0][2]) {
+            double d1, d2;
+            if (IS_E) {
+                d1 = (double(s2[idx + sy]) - double(c2)) * inv_dx;
+                d2 = (double(ny) - double(cy)) * inv_dx;
+            } else {
+                d1 = (double(s2[idx - sy]) - double(c2)) * inv_dx;
+                d2 = (double(cy) - double(ny)) * inv_dx;
+            }
+            update_point<T, MAT, 0>(A, i, j, k, idx, d1, d2);
+        }
+        // component y: d/dz of s0, d/dx of s2
+        if (in1 && k >= A.lo[1][2] && k <= A.hi[1][2]) {
+            double d1, d2;
+            if (IS_E) {
+                d1 = (double(nx) - double(cx)) * inv_dx;
+                d2 = (double(s2[idx + 1]) - double(c2)) * inv_dx;
+            } else {
+                d1 = (double(cx) - double(nx)) * inv_dx;
+                d2 = (double(s2[idx - 1]) - double(c2)) * inv_dx;
+            }
+            update_point<T, MAT, 1>(A, i, j, k, idx, d1, d2);
+        }
+        // component z: d/dx of s1, d/dy of s0
+        if (in2 && k >= A.lo[2][2] && k <= A.hi[2][2]) {
+            double d1, d2;
+            if (IS_E) {
+                d1 = (double(s1[idx + 1]) - double(cy)) * inv_dx;
+                d2 = (double(s0[idx + sy]) - double(cx)) * inv_dx;
+            } else {
+                d1 = (double(s1[idx - 1]) - double(ny)) * inv_dx;
+                d2 = (double(s0[idx - sy]) - double(nx)) * inv_dx;
+            }
+            update_point<T, MAT, 2>(A, i, j, k, idx, d1, d2);
+        }
+        cx = nx;
+        cy = ny;
+    }
+}
+
+// ---------------------------------------------------------------- sources
+template <typename T>
+struct SrcArgs {
+    int ntarget;
+    T* u[3];
+    const int64_t* idx;
+    const int* comp;
+    const double* scale;  // dt / g_inf of the target point
+    const int* eoff;      // [ntarget+1] into entries
+    const double* amp;
+    const double* start;
+    const double* end;
+    const double* width;
+    const int* moff;      // [ntarget+1] into monitor hits
+    const int* munit;
+    const int64_t* mofs;
+    double dt;
+    DftDev units[MAXU];
+    const double* ph;
+};
+
+template <typename T>
+__global__ void k_sources(const SrcArgs<T> A, double t)
+{
+    const int q = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    if (q >= A.ntarget)
+        return;
+    double g = 0;
+    for (int e = A.eoff[q]; e < A.eoff[q + 1]; ++e)
+        if (t > A.start[e] && t < A.end[e])
+            g += A.amp[e] * gaussian1(t - (A.start[e] + A.end[e]) / 2, A.width[e]);
+    if (g == 0)
+        return;
+    T* f = A.u[A.comp[q]];
+    const int64_t i = A.idx[q];
+    const T old = f[i];
+    const T nv = T(double(old) + g * A.scale[q]);
+    f[i] = nv;
+    const double dv = double(nv) - double(old);
+    const double dD = g * A.dt;
+    for (int m = A.moff[q]; m < A.moff[q + 1]; ++m) {
+        const DftDev& U = A.units[A.munit[m]];
+        const double v = U.is_db ? dD : dv;
+        const double* ph = A.ph + 2 * U.mf0;
+        for (int fq = 0; fq < U.nf; ++fq) {
+            const int64_t o = int64_t(fq) * U.npts + A.mofs[m];
+            U.im[o] += ph[2 * fq + 1] * v;
+            U.re[o] += ph[2 * fq] * v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- ghost faces
+struct FaceDev {
+    int comp, axis, dst, src;
+    int t0lo, t0hi, t1lo, t1hi;  // transverse ranges (axes (axis+1)%3, (axis+2)%3)
+    double mult;
+};
+static constexpr int MAXF = 16;
+template <typename T>
+struct FaceArgs {
+    T* u[3];
+    int64_t pitch, plane;
+    int nf;
+    FaceDev f[MAXF];
+};
+
+template <typename T>
+__global__ void k_faces(const FaceArgs<T> A)
+{
+    const FaceDev& F = A.f[blockIdx.z];
+    const int t0 = F.t0lo + int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    const int t1 = F.t1lo + int(blockIdx.y);
+    if (t0 > F.t0hi || t1 > F.t1hi)
+        return;
+    int c[3];
+    const int a0 = (F.axis + 1) % 3, a1 = (F.axis + 2) % 3;
+    c[a0] = t0;
+    c[a1] = t1;
+    c[F.axis] = F.src;
+    const int64_t s = c[0] + A.pitch * c[1] + A.plane * int64_t(c[2]);
+    c[F.axis] = F.dst;
+    const int64_t d = c[0] + A.pitch * c[1] + A.plane * int64_t(c[2]);
+    T* u = A.u[F.comp % 3];
+    u[d] = T(F.mult * double(u[s]));
+}
+
+// ---------------------------------------------------------------- probes / phasors
+template <typename T>
+struct ProbeArgs {
+    const T* u[6];
+    int np;
+    const int* off;
+    const int* comp;
+    const int64_t* idx;
+    const double* w;
+    double* trace;
+};
+
+template <typename T>
+__global__ void k_probes(const ProbeArgs<T> A, int64_t row)
+{
+    const int p = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    if (p >= A.np)
+        return;
+    double v = 0;
+    for (int q = A.off[p]; q < A.off[p + 1]; ++q)
+        v += double(A.u[A.comp[q]][A.idx[q]]) * A.w[q];
+    A.trace[row * A.np + p] = v;
+}
+
+// table[s][mf][2] for steps step0 .. step0+count-1: time = (step + toff) * dt
+__global__ void k_phasors(double* table, const double* freqs, int mf, int64_t step0, int count, double toff, double dt)
+{
+    const int q = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);
+    if (q >= mf * count)
+        return;
+    const int s = q / mf, f = q % mf;
+    const double pi = 3.141592653589793e+00;
+    const double time = (double(step0 + s) + toff) * dt;
+    const double arg = 2 * pi * freqs[f] * time;
+    table[2 * int64_t(q)] = cos(arg);
+    table[2 * int64_t(q) + 1] = -sin(arg);
+}
+
+// ---------------------------------------------------------------- engine
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct EngineImpl {
+    std::unique_ptr<Setup> S;
+    bool fp32 = false;
+    size_t tsz = 8;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    double bytes = 0;
+    int64_t n = 0;  // time_step
+
+    void* field[6] = {};
+    void* Pa[2] = {nullptr, nullptr};
+    void* Pb[2] = {nullptr, nullptr};
+    int S_slots[2] = {0, 0};
+    uint8_t* ids[2][3] = {};
+    double *m_inv[2] = {}, *m_eps[2] = {}, *p_c1[2] = {}, *p_c2[2] = {}, *p_sc3[2] = {};
+    int *m_np[2] = {}, *m_p0[2] = {};
+    double *d_inv[2][3] = {}, *d_eps[2][3] = {}, *d_amp[2][3] = {}, *d_c[2][3] = {};
+    PmlDev pml[2][3] = {};
+    std::vector<DftDev> units[2];
+    std::vector<std::vector<double>> host_freqs[2];
+    double* freqs_dev[2] = {};
+    int MF[2] = {0, 0};
+    double* ph[2] = {};
+    int64_t ph_start = -1;
+    // sources
+    struct Src {
+        int ntarget = 0;
+        int64_t* idx = nullptr;
+        int *comp = nullptr, *eoff = nullptr, *moff = nullptr, *munit = nullptr;
+        double *scale = nullptr, *amp = nullptr, *start = nullptr, *end = nullptr, *width = nullptr;
+        int64_t* mofs = nullptr;
+        double max_end = -1;
+    } src[2];
+    // faces
+    std::vector<FaceDev> faces[2];
+    // probes
+    int nprobes = 0;
+    int *pr_off = nullptr, *pr_comp = nullptr;
+    int64_t* pr_idx = nullptr;
+    double* pr_w = nullptr;
+    double* trace = nullptr;
+    int64_t trace_cap = 0;
+
+    template <typename X>
+    X* dalloc(size_t count, bool zero = true)
+    {
+        if (count == 0)
+            count = 1;
+        void* p = nullptr;
+        HIP_OK(hipMalloc(&p, count * sizeof(X)));
+        allocs.push_back(p);
+        bytes += double(count * sizeof(X));
+        if (zero)
+            HIP_OK(hipMemsetAsync(p, 0, count * sizeof(X), stream));
+        return static_cast<X*>(p);
+    }
+    template <typename X>
+    X* upload(const std::vector<X>& v)
+    {
+        // synchronous copy: the host vector may be a temporary
+        X* p = dalloc<X>(v.size(), false);
+        if (!v.empty())
+            HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+        else {
+            X zero{};
+            HIP_OK(hipMemcpy(p, &zero, sizeof(X), hipMemcpyHostToDevice));
+        }
+        return p;
+    }
+
+    void build();
+    template <typename T>
+    HalfArgs<T> half_args(int h);
+    template <typename T>
+    void launch_half(int h, hipEvent_t e0, hipEvent_t e1);
+    void ensure_phasors();
+    template <typename T>
+    void step_once(hipEvent_t* ev, bool timed_e, bool timed_h);
+    void step_n(int64_t nsteps);
+    ~EngineImpl();
+};
+
+EngineImpl::~EngineImpl()
+{
+    if (stream)
+        hipStreamSynchronize(stream);
+    for (void* p : allocs)
+        hipFree(p);
+    if (stream)
+        hipStreamDestroy(stream);
+}
+
+void EngineImpl::build()
+{
+    Setup& s = *S;
+    const Layout& L = s.L;
+    fp32 = s.precision == "fp32";
+    tsz = fp32 ? 4 : 8;
+    HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (int c = 0; c < 6; ++c) {
+        field[c] = fp32 ? (void*)dalloc<float>(L.alloc) : (void*)dalloc<double>(L.alloc);
+    }
+    for (int h = 0; h < 2; ++h) {
+        HalfMaterial& M = s.mat[h];
+        S_slots[h] = M.uniform ? 0 : M.max_poles;
+        if (S_slots[h] > MAXS)
+            throw std::runtime_error("more than 8 polarization poles at one point");
+        if (S_slots[h] > 0) {
+            size_t cnt = size_t(S_slots[h]) * 3 * size_t(L.alloc);
+            Pa[h] = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
+            Pb[h] = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
+        }
+        if (!M.uniform) {
+            for (int c = 0; c < 3; ++c)
+                ids[h][c] = upload(M.ids[c]);
+            m_inv[h] = upload(M.table.inv_eps);
+            m_eps[h] = upload(M.table.eps);
+            m_np[h] = upload(M.table.npoles);
+            m_p0[h] = upload(M.table.pole0);
+            p_c1[h] = upload(M.table.c1);
+            p_c2[h] = upload(M.table.c2);
+            p_sc3[h] = upload(M.table.sc3);
+            for (int c = 0; c < 3; ++c) {
+                DenseRegion& D = M.dense[c];
+                if (!D.present)
+                    continue;
+                d_inv[h][c] = upload(D.inv_eps);
+                d_eps[h][c] = upload(D.eps);
+                d_amp[h][c] = upload(D.amp);
+                std::vector<double> cc;
+                for (int q = 0; q < D.npoles; ++q) {
+                    cc.push_back(D.c1[q]);
+                    cc.push_back(D.c2[q]);
+                    cc.push_back(D.c3[q]);
+                }
+                d_c[h][c] = upload(cc);
+            }
+        }
+        // CPML: E points use even-parity tables, H points odd-parity ones
+        const int par = h == 0 ? 0 : 1;
+        for (int a = 0; a < 3; ++a) {
+            PmlAxis& P = s.pml[a];
+            PmlDev& D = pml[h][a];
+            D.W = P.W[par];
+            if (D.W == 0)
+                continue;
+            D.smap = upload(P.smap[par]);
+            D.b = upload(P.b[par]);
+            D.c = upload(P.c[par]);
+            D.ik = upload(P.ik[par]);
+            size_t cnt = size_t(D.W);
+            for (int b = 0; b < 3; ++b)
+                if (b != a)
+                    cnt *= size_t(L.n[b]);
+            D.psi1 = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
+            D.psi2 = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
+        }
+        // DFT units
+        std::vector<double> fl;
+        for (auto& U : s.units[h]) {
+            DftDev d{};
+            d.comp = U.comp % 3;
+            d.is_db = U.is_db ? 1 : 0;
+            d.nf = int(U.freqs.size());
+            d.mf0 = int(fl.size());
+            for (double f : U.freqs)
+                fl.push_back(f);
+            if (U.has_local) {
+                for (int a = 0; a < 3; ++a) {
+                    d.box[a][0] = U.box[a][0];
+                    d.box[a][1] = U.box[a][1];
+                    d.dims[a] = U.dims[a];
+                }
+                d.npts = U.npts();
+            } else {
+                for (int a = 0; a < 3; ++a) {
+                    d.box[a][0] = 1;
+                    d.box[a][1] = 0;
+                    d.dims[a] = 0;
+                }
+                d.npts = 0;
+            }
+            d.re = dalloc<double>(size_t(d.npts) * d.nf);
+            d.im = dalloc<double>(size_t(d.npts) * d.nf);
+            units[h].push_back(d);
+        }
+        if (units[h].size() > size_t(MAXU))
+            throw std::runtime_error("more than 16 DFT units on one half step");
+        MF[h] = int(fl.size());
+        freqs_dev[h] = upload(fl);
+        ph[h] = dalloc<double>(size_t(std::max(1, MF[h])) * 2 * PH_CHUNK);
+        // sources: group dipoles per target point (insertion order kept)
+        std::vector<std::pair<int64_t, int>> key;  // (index, comp)
+        std::vector<std::vector<int>> members;
+        std::unordered_map<int64_t, size_t> where;
+        for (size_t q = 0; q < s.dip[h].size(); ++q) {
+            const Dipole& D = s.dip[h][q];
+            int64_t idx = L.index(D.i, D.j, D.k) - L.origin;
+            int64_t hk = idx * 4 + D.comp % 3;
+            auto it = where.find(hk);
+            size_t t;
+            if (it == where.end()) {
+                t = key.size();
+                where.emplace(hk, t);
+                key.emplace_back(idx, D.comp % 3);
+                members.emplace_back();
+            } else
+                t = it->second;
+            members[t].push_back(int(q));
+        }
+        Src& SR = src[h];
+        SR.ntarget = int(key.size());
+        if (SR.ntarget > 0) {
+            std::vector<int64_t> vidx;
+            std::vector<int> vcomp, eoff{0}, moff{0}, munit;
+            std::vector<double> scale, amp, st, en, wd;
+            std::vector<int64_t> mofs;
+            for (size_t t = 0; t < key.size(); ++t) {
+                vidx.push_back(key[t].first);
+                vcomp.push_back(key[t].second);
+                const Dipole& D0 = s.dip[h][members[t][0]];
+                // point's g_inf
+                double g;
+                if (M.uniform)
+                    g = M.u_eps;
+                else {
+                    int id = M.ids[key[t].second][key[t].first + L.origin];
+                    if (id != 255)
+                        g = M.table.eps[id];
+                    else {
+                        DenseRegion& DR = M.dense[key[t].second];
+                        int64_t d0 = DR.box[0][1] - DR.box[0][0] + 1, d1 = DR.box[1][1] - DR.box[1][0] + 1;
+                        g = DR.eps[(int64_t(D0.k - DR.box[2][0]) * d1 + (D0.j - DR.box[1][0])) * d0 +
+                                   (D0.i - DR.box[0][0])];
+                    }
+                }
+                scale.push_back(s.dt / g);
+                for (int q : members[t]) {
+                    const Dipole& D = s.dip[h][q];
+                    amp.push_back(D.amp);
+                    st.push_back(D.start);
+                    en.push_back(D.end);
+                    wd.push_back(D.width);
+                    SR.max_end = std::max(SR.max_end, D.end);
+                }
+                eoff.push_back(int(amp.size()));
+                for (size_t u = 0; u < s.units[h].size(); ++u) {
+                    const DftUnit& U = s.units[h][u];
+                    if (!U.has_local || U.comp % 3 != key[t].second)
+                        continue;
+                    if (D0.i < U.box[0][0] || D0.i > U.box[0][1] || D0.j < U.box[1][0] || D0.j > U.box[1][1] ||
+                        D0.k < U.box[2][0] || D0.k > U.box[2][1])
+                        continue;
+                    munit.push_back(int(u));
+                    mofs.push_back((D0.i - U.box[0][0]) +
+                                   int64_t(U.dims[0]) * ((D0.j - U.box[1][0]) + int64_t(U.dims[1]) * (D0.k - U.box[2][0])));
+                }
+                moff.push_back(int(munit.size()));
+            }
+            SR.idx = upload(vidx);
+            SR.comp = upload(vcomp);
+            SR.scale = upload(scale);
+            SR.eoff = upload(eoff);
+            SR.amp = upload(amp);
+            SR.start = upload(st);
+            SR.end = upload(en);
+            SR.width = upload(wd);
+            SR.moff = upload(moff);
+            SR.munit = upload(munit);
+            SR.mofs = upload(mofs);
+        }
+        // faces
+        for (auto& F : s.faces[h]) {
+            FaceDev d{};
+            d.comp = F.comp;
+            d.axis = F.axis;
+            d.dst = F.dst;
+            d.src = F.src;
+            d.mult = F.mult;
+            int a0 = (F.axis + 1) % 3, a1 = (F.axis + 2) % 3;
+            d.t0lo = L.lo[F.comp][a0];
+            d.t0hi = L.hi[F.comp][a0];
+            d.t1lo = L.lo[F.comp][a1];
+            d.t1hi = L.hi[F.comp][a1];
+            if (d.t0hi >= d.t0lo && d.t1hi >= d.t1lo)
+                faces[h].push_back(d);
+        }
+        if (faces[h].size() > size_t(MAXF))
+            throw std::runtime_error("too many boundary faces");
+    }
+    // probes
+    nprobes = int(s.probes.size());
+    if (nprobes) {
+        std::vector<int> off{0}, comp;
+        std::vector<int64_t> idx;
+        std::vector<double> w;
+        for (auto& P : s.probes) {
+            for (auto& t : P) {
+                comp.push_back(t.comp);
+                idx.push_back(t.index - L.origin);
+                w.push_back(t.w);
+            }
+            off.push_back(int(idx.size()));
+        }
+        pr_off = upload(off);
+        pr_comp = upload(comp);
+        pr_idx = upload(idx);
+        pr_w = upload(w);
+    }
+    HIP_OK(hipStreamSynchronize(stream));
+}
+
+template <typename T>
+HalfArgs<T> EngineImpl::half_args(int h)
+{
+    const Setup& s = *S;
+    const Layout& L = s.L;
+    HalfArgs<T> A;
+    std::memset(&A, 0, sizeof(A));
+    for (int c = 0; c < 3; ++c) {
+        A.u[c] = static_cast<T*>(field[h * 3 + c]) + L.origin;
+        A.s[c] = static_cast<const T*>(field[(1 - h) * 3 + c]) + L.origin;
+    }
+    A.pitch = L.pitch;
+    A.plane = L.plane;
+    for (int a = 0; a < 3; ++a) {
+        A.n[a] = L.n[a];
+        A.dlo[a] = 1 << 30;
+        A.dhi[a] = -1;
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) {
+            A.lo[c][a] = L.lo[h * 3 + c][a];
+            A.hi[c][a] = L.hi[h * 3 + c][a];
+            if (A.hi[c][a] >= A.lo[c][a]) {
+                A.dlo[a] = std::min(A.dlo[a], A.lo[c][a]);
+                A.dhi[a] = std::max(A.dhi[a], A.hi[c][a]);
+            }
+        }
+    A.inv_dx = 1 / s.dx;
+    A.dt = s.dt;
+    const HalfMaterial& M = s.mat[h];
+    A.u_inv = M.u_inv;
+    A.u_eps = M.u_eps;
+    if (!M.uniform) {
+        for (int c = 0; c < 3; ++c)
+            A.ids[c] = ids[h][c] + L.origin;
+        A.m_inv = m_inv[h];
+        A.m_eps = m_eps[h];
+        A.m_np = m_np[h];
+        A.m_p0 = m_p0[h];
+        A.p_c1 = p_c1[h];
+        A.p_c2 = p_c2[h];
+        A.p_sc3 = p_sc3[h];
+        for (int c = 0; c < 3; ++c) {
+            const DenseRegion& D = M.dense[c];
+            A.d_present[c] = D.present ? 1 : 0;
+            if (!D.present)
+                continue;
+            for (int a = 0; a < 3; ++a) {
+                A.d_box[c][a][0] = D.box[a][0];
+                A.d_box[c][a][1] = D.box[a][1];
+            }
+            A.d_np[c] = D.npoles;
+            A.d_inv[c] = d_inv[h][c];
+            A.d_eps[c] = d_eps[h][c];
+            A.d_amp[c] = d_amp[h][c];
+            A.d_c[c] = d_c[h][c];
+        }
+    }
+    A.pstride = L.alloc;
+    if (S_slots[h] > 0) {
+        A.Pcur = static_cast<T*>(Pa[h]) + L.origin;
+        A.Pprev = static_cast<T*>(Pb[h]) + L.origin;
+    }
+    for (int a = 0; a < 3; ++a)
+        A.pml[a] = pml[h][a];
+    A.nunits = int(units[h].size());
+    for (int u = 0; u < A.nunits; ++u)
+        A.units[u] = units[h][u];
+    A.ph = ph[h] + size_t(n - ph_start) * 2 * std::max(1, MF[h]);
+    return A;
+}
+
+void EngineImpl::ensure_phasors()
+{
+    if (ph_start >= 0 && n < ph_start + PH_CHUNK)
+        return;
+    ph_start = n;
+    for (int h = 0; h < 2; ++h) {
+        if (MF[h] == 0)
+            continue;
+        int total = MF[h] * PH_CHUNK;
+        // H units sample at (n + 1/2) dt, E units at (n + 1) dt (src/simulation.cpp:748-766)
+        k_phasors<<<(total + 255) / 256, 256, 0, stream>>>(ph[h], freqs_dev[h], MF[h], ph_start, PH_CHUNK,
+                                                         h == 0 ? 1.0 : 0.5, S->dt);
+        HIP_OK(hipGetLastError());
+    }
+}
+
+template <typename T>
+void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
+{
+    HalfArgs<T> A = half_args<T>(h);
+    if (A.dhi[0] < A.dlo[0] || A.dhi[1] < A.dlo[1] || A.dhi[2] < A.dlo[2])
+        return;
+    dim3 block(BX, BY, 1);
+    dim3 grid((A.dhi[0] - A.dlo[0] + BX) / BX, (A.dhi[1] - A.dlo[1] + BY) / BY, (A.dhi[2] - A.dlo[2] + LZ) / LZ);
+    const bool mat = !S->mat[h].uniform;
+    if (e0)
+        HIP_OK(hipEventRecord(e0, stream));
+    if (h == 0) {
+        if (mat)
+            k_half<T, true, true><<<grid, block, 0, stream>>>(A);
+        else
+            k_half<T, true, false><<<grid, block, 0, stream>>>(A);
+    } else {
+        if (mat)
+            k_half<T, false, true><<<grid, block, 0, stream>>>(A);
+        else
+            k_half<T, false, false><<<grid, block, 0, stream>>>(A);
+    }
+    HIP_OK(hipGetLastError());
+    if (e1)
+        HIP_OK(hipEventRecord(e1, stream));
+    if (S_slots[h] > 0)
+        std::swap(Pa[h], Pb[h]);
+    // sources of this half at its source time (src/simulation.cpp:748-766)
+    const double t = h == 1 ? double(n) * S->dt : (double(n) + 0.5) * S->dt;
+    Src& SR = src[h];
+    if (SR.ntarget > 0 && !(t > SR.max_end)) {
+        SrcArgs<T> B;
+        std::memset(&B, 0, sizeof(B));
+        B.ntarget = SR.ntarget;
+        for (int c = 0; c < 3; ++c)
+            B.u[c] = static_cast<T*>(field[h * 3 + c]) + S->L.origin;
+        B.idx = SR.idx;
+        B.comp = SR.comp;
+        B.scale = SR.scale;
+        B.eoff = SR.eoff;
+        B.amp = SR.amp;
+        B.start = SR.start;
+        B.end = SR.end;
+        B.width = SR.width;
+        B.moff = SR.moff;
+        B.munit = SR.munit;
+        B.mofs = SR.mofs;
+        B.dt = S->dt;
+        for (size_t u = 0; u < units[h].size(); ++u)
+            B.units[u] = units[h][u];
+        B.ph = A.ph;
+        k_sources<T><<<(SR.ntarget + 255) / 256, 256, 0, stream>>>(B, t);
+        HIP_OK(hipGetLastError());
+    }
+    if (!faces[h].empty()) {
+        FaceArgs<T> F;
+        std::memset(&F, 0, sizeof(F));
+        for (int c = 0; c < 3; ++c)
+            F.u[c] = static_cast<T*>(field[h * 3 + c]) + S->L.origin;
+        F.pitch = S->L.pitch;
+        F.plane = S->L.plane;
+        F.nf = int(faces[h].size());
+        int m0 = 1, m1 = 1;
+        for (int q = 0; q < F.nf; ++q) {
+            F.f[q] = faces[h][q];
+            m0 = std::max(m0, F.f[q].t0hi - F.f[q].t0lo + 1);
+            m1 = std::max(m1, F.f[q].t1hi - F.f[q].t1lo + 1);
+        }
+        k_faces<T><<<dim3((m0 + 255) / 256, m1, F.nf), 256, 0, stream>>>(F);
+        HIP_OK(hipGetLastError());
+    }
+}
+
+template <typename T>
+void EngineImpl::step_once(hipEvent_t* ev, bool timed_e, bool timed_h)
+{
+    ensure_phasors();
+    launch_half<T>(1, timed_h ? ev[2] : nullptr, timed_h ? ev[3] : nullptr);
+    launch_half<T>(0, timed_e ? ev[0] : nullptr, timed_e ? ev[1] : nullptr);
+    if (nprobes && n < trace_cap) {
+        ProbeArgs<T> P;
+        std::memset(&P, 0, sizeof(P));
+        for (int c = 0; c < 6; ++c)
+            P.u[c] = static_cast<const T*>(field[c]) + S->L.origin;
+        P.np = nprobes;
+        P.off = pr_off;
+        P.comp = pr_comp;
+        P.idx = pr_idx;
+        P.w = pr_w;
+        P.trace = trace;
+        k_probes<T><<<(nprobes + 63) / 64, 64, 0, stream>>>(P, n);
+        HIP_OK(hipGetLastError());
+    }
+    ++n;
+}
+
+void EngineImpl::step_n(int64_t nsteps)
+{
+    for (int64_t q = 0; q < nsteps; ++q) {
+        if (fp32)
+            step_once<float>(nullptr, false, false);
+        else
+            step_once<double>(nullptr, false, false);
+    }
+}
+
+// ---------------------------------------------------------------- Engine API
+Engine::Engine(std::unique_ptr<Setup> setup) : p_(new EngineImpl)
+{
+    p_->S = std::move(setup);
+    p_->build();
+}
+
+Engine::~Engine() = default;
+
+const Setup& Engine::setup() const { return *p_->S; }
+int64_t Engine::time_step() const { return p_->n; }
+
+void Engine::sync() { HIP_OK(hipStreamSynchronize(p_->stream)); }
+double Engine::device_bytes() const { return p_->bytes; }
+
+void Engine::step(int64_t nsteps)
+{
+    p_->step_n(nsteps);
+    sync();
+}
+
+int64_t Engine::run_until_time(double t)
+{
+    // while (time_step * dt < time), src/simulation.cpp:768-780
+    int64_t taken = 0;
+    while (double(p_->n) * p_->S->dt < t) {
+        p_->step_n(1);
+        ++taken;
+    }
+    sync();
+    return taken;
+}
+
+KernelTiming Engine::step_timed(int64_t nsteps, bool per_kernel)
+{
+    KernelTiming r;
+    hipEvent_t t0, t1;
+    HIP_OK(hipEventCreate(&t0));
+    HIP_OK(hipEventCreate(&t1));
+    std::vector<hipEvent_t> ev;
+    if (per_kernel) {
+        ev.resize(size_t(nsteps) * 4);
+        for (auto& e : ev)
+            HIP_OK(hipEventCreate(&e));
+    }
+    sync();
+    HIP_OK(hipEventRecord(t0, p_->stream));
+    for (int64_t q = 0; q < nsteps; ++q) {
+        hipEvent_t* e = per_kernel ? &ev[size_t(q) * 4] : nullptr;
+        if (p_->fp32)
+            p_->step_once<float>(e, per_kernel, per_kernel);
+        else
+            p_->step_once<double>(e, per_kernel, per_kernel);
+    }
+    HIP_OK(hipEventRecord(t1, p_->stream));
+    HIP_OK(hipEventSynchronize(t1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, t0, t1));
+    r.total_ms = ms;
+    r.steps = nsteps;
+    if (per_kernel) {
+        for (int64_t q = 0; q < nsteps; ++q) {
+            float a = 0, b = 0;
+            HIP_OK(hipEventElapsedTime(&a, ev[size_t(q) * 4 + 0], ev[size_t(q) * 4 + 1]));
+            HIP_OK(hipEventElapsedTime(&b, ev[size_t(q) * 4 + 2], ev[size_t(q) * 4 + 3]));
+            r.e_ms += a;
+            r.h_ms += b;
+        }
+        for (auto& e : ev)
+            hipEventDestroy(e);
+    }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return r;
+}
+
+void Engine::dft_dims(int k, int64_t dims[4]) const
+{
+    const Setup& s = *p_->S;
+    if (k < 0 || k >= int(s.outputs.size()))
+        throw std::runtime_error("DFT output index out of range");
+    auto hu = s.output_units[k];
+    const DftUnit& U = s.units[hu.first][hu.second];
+    dims[0] = int64_t(s.outputs[k].freqs.size());
+    dims[1] = (U.q2[2] - U.q1[2]) / 2 + 1;
+    dims[2] = (U.q2[1] - U.q1[1]) / 2 + 1;
+    dims[3] = (U.q2[0] - U.q1[0]) / 2 + 1;
+}
+
+// output_fields_collective (src/dft.cpp:273-320) for one rank
+void Engine::get_dft(int k, double* re, double* im, double* x, double* y, double* z, double* f)
+{
+    sync();
+    const Setup& s = *p_->S;
+    int64_t dims[4];
+    dft_dims(k, dims);
+    auto hu = s.output_units[k];
+    const DftUnit& U = s.units[hu.first][hu.second];
+    const DftOutput& O = s.outputs[k];
+    const DftDev& D = p_->units[hu.first][hu.second];
+    const int64_t full = dims[1] * dims[2] * dims[3];
+    if (re)
+        std::fill(re, re + dims[0] * full, 0.0);
+    if (im)
+        std::fill(im, im + dims[0] * full, 0.0);
+    if (U.has_local && D.npts > 0) {
+        std::vector<double> hre(size_t(D.npts) * D.nf), him(size_t(D.npts) * D.nf);
+        HIP_OK(hipMemcpy(hre.data(), D.re, hre.size() * 8, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(him.data(), D.im, him.size() * 8, hipMemcpyDeviceToHost));
+        int off[3];
+        for (int a = 0; a < 3; ++a)
+            off[a] = (U.l1[a] - U.q1[a]) / 2;
+        for (size_t fi = 0; fi < O.freqs.size(); ++fi) {
+            auto it = std::lower_bound(U.freqs.begin(), U.freqs.end(), O.freqs[fi]);
+            if (it == U.freqs.end() || *it != O.freqs[fi])
+                continue;
+            int64_t fu = it - U.freqs.begin();
+            for (int kk = 0; kk < U.dims[2]; ++kk)
+                for (int jj = 0; jj < U.dims[1]; ++jj)
+                    for (int ii = 0; ii < U.dims[0]; ++ii) {
+                        int64_t src = fu * D.npts + ii + int64_t(U.dims[0]) * (jj + int64_t(U.dims[1]) * kk);
+                        int64_t dst = int64_t(fi) * full + (ii + off[0]) +
+                                      dims[3] * ((jj + off[1]) + dims[2] * (kk + off[2]));
+                        if (re)
+                            re[dst] = hre[src];
+                        if (im)
+                            im[dst] = him[src];
+                    }
+        }
+    }
+    // linspace of the closure (src/dft.cpp:312-318, src/utility.cpp:281-291)
+    auto lin = [](double a, double b, int64_t n, double* out) {
+        if (!out)
+            return;
+        if (n <= 1) {
+            out[0] = a;
+            return;
+        }
+        double st = (b - a) / double(n - 1);
+        for (int64_t q = 0; q < n; ++q)
+            out[q] = a + double(q) * st;
+    };
+    lin(U.q1[0] * s.dx2, U.q2[0] * s.dx2, dims[3], x);
+    lin(U.q1[1] * s.dx2, U.q2[1] * s.dx2, dims[2], y);
+    lin(U.q1[2] * s.dx2, U.q2[2] * s.dx2, dims[1], z);
+    if (f)
+        for (size_t q = 0; q < O.freqs.size(); ++q)
+            f[q] = O.freqs[q];
+}
+
+void Engine::set_trace_capacity(int64_t steps)
+{
+    if (p_->nprobes == 0 || steps <= p_->trace_cap)
+        return;
+    sync();
+    double* t = p_->dalloc<double>(size_t(steps) * p_->nprobes);
+    if (p_->trace && p_->trace_cap)
+        HIP_OK(hipMemcpyAsync(t, p_->trace, size_t(p_->trace_cap) * p_->nprobes * 8, hipMemcpyDeviceToDevice,
+                              p_->stream));
+    p_->trace = t;
+    p_->trace_cap = steps;
+    sync();
+}
+
+int64_t Engine::trace_len() const { return std::min(p_->n, p_->trace_cap); }
+
+void Engine::get_trace(double* out, int64_t cap)
+{
+    sync();
+    int64_t n = std::min(trace_len(), cap);
+    if (n > 0 && p_->nprobes)
+        HIP_OK(hipMemcpy(out, p_->trace, size_t(n) * p_->nprobes * 8, hipMemcpyDeviceToHost));
+}
+
+void Engine::component_dims(int ct, int64_t dims[3]) const
+{
+    const Layout& L = p_->S->L;
+    int c = comp_index(ct);
+    dims[0] = std::max(0, L.hi[c][2] - L.lo[c][2] + 1);
+    dims[1] = std::max(0, L.hi[c][1] - L.lo[c][1] + 1);
+    dims[2] = std::max(0, L.hi[c][0] - L.lo[c][0] + 1);
+}
+
+void Engine::get_component(int ct, double* out)
+{
+    sync();
+    const Layout& L = p_->S->L;
+    int c = comp_index(ct);
+    int64_t dims[3];
+    component_dims(ct, dims);
+    if (dims[0] * dims[1] * dims[2] == 0)
+        return;
+    size_t ts = p_->tsz;
+    std::vector<char> host(size_t(L.alloc) * ts);
+    HIP_OK(hipMemcpy(host.data(), p_->field[c], host.size(), hipMemcpyDeviceToHost));
+    int64_t q = 0;
+    for (int k = L.lo[c][2]; k <= L.hi[c][2]; ++k)
+        for (int j = L.lo[c][1]; j <= L.hi[c][1]; ++j)
+            for (int i = L.lo[c][0]; i <= L.hi[c][0]; ++i) {
+                int64_t id = L.index(i, j, k);
+                out[q++] = p_->fp32 ? double(reinterpret_cast<float*>(host.data())[id])
+                                    : reinterpret_cast<double*>(host.data())[id];
+            }
+}
+
+double Engine::probe(const double pos[3], int ct)
+{
+    sync();
+    auto terms = probe_terms(*p_->S, pos, ct);
+    double v = 0;
+    for (auto& t : terms) {
+        double x;
+        if (p_->fp32) {
+            float h;
+            HIP_OK(hipMemcpy(&h, static_cast<float*>(p_->field[t.comp]) + t.index, 4, hipMemcpyDeviceToHost));
+            x = h;
+        } else {
+            HIP_OK(hipMemcpy(&x, static_cast<double*>(p_->field[t.comp]) + t.index, 8, hipMemcpyDeviceToHost));
+        }
+        v += x * t.w;
+    }
+    return v;
+}
+
+int64_t Engine::run_until_dft(const double*, const double*, int, double, double, double, std::string*)
+{
+    throw std::runtime_error("run_until_dft is not available yet");
+}
+
+int64_t Engine::run_until_decay(const double*, int, double, double, std::string*)
+{
+    throw std::runtime_error("run_until_fields_decayed is not available yet");
+}
+
+}  // namespace ffip_amd

@@ -1,0 +1,61 @@
+// Device engine: owns the device state of one simulation and advances it.
+// Pure C++ interface (no HIP types) so the C-ABI layer stays plain.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "setup.hpp"
+
+namespace ffip_amd {
+
+struct EngineImpl;
+
+struct KernelTiming {
+    double total_ms = 0;  // wall time of the timed region (HIP events)
+    double e_ms = 0;      // summed duration of the electric half-step kernel
+    double h_ms = 0;      // summed duration of the magnetic half-step kernel
+    int64_t steps = 0;
+};
+
+class Engine {
+public:
+    explicit Engine(std::unique_ptr<Setup> setup);
+    ~Engine();
+
+    const Setup& setup() const;
+    int64_t time_step() const;
+
+    void step(int64_t nsteps);
+    int64_t run_until_time(double t);
+    // run_until_dft / run_until_fields_decayed (src/simulation.cpp:782-901)
+    int64_t run_until_dft(const double c[3], const double s[3], int ct, double freq, double interval, double var,
+                          std::string* log);
+    int64_t run_until_decay(const double pos[3], int ct, double interval, double decay_by, std::string* log);
+
+    // timed stepping for measurement (events on the engine's stream)
+    KernelTiming step_timed(int64_t nsteps, bool per_kernel);
+
+    // DFT output k (the k-th "fields output" entry): real/imag [nf, nz, ny, nx]
+    // over the closure, coordinates and frequencies.
+    void dft_dims(int k, int64_t dims[4]) const;
+    void get_dft(int k, double* re, double* im, double* x, double* y, double* z, double* f);
+    // probe traces [steps][nprobes]
+    int64_t trace_len() const;
+    void get_trace(double* out, int64_t cap);
+    void set_trace_capacity(int64_t steps);
+    // Simulation::at (include/utility.hpp:983-1022) for an E/H component
+    double probe(const double pos[3], int ct);
+    // raw component values on the component lattice, [nz][ny][nx]
+    void component_dims(int ct, int64_t dims[3]) const;
+    void get_component(int ct, double* out);
+    void sync();
+    double device_bytes() const;
+
+private:
+    std::unique_ptr<EngineImpl> p_;
+};
+
+}  // namespace ffip_amd

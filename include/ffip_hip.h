@@ -1,0 +1,108 @@
+/*
+ * ffip_hip.h -- C ABI of libffip_hip.so, the MI355X (gfx950) FDTD engine that
+ * replaces the time-stepping core of JoeZeng133/ffip.
+ *
+ * The reference crosses into its C++ core through a process and file boundary:
+ * ffip.Simulation.run (ffip/simulation.py:404-431) writes config.json plus an
+ * input HDF5 file, runs `mpirun -np N run_sim_json` (app/run_sim_json.cpp:6-25,
+ * which calls Simulation::init / run / output, include/simulation.hpp:104-167)
+ * and reads the DFT groups back from the output HDF5 file.  This library offers
+ * the same steps in-process: the JSON document is the one ffip's Python layer
+ * writes, the input datasets are handed over as arrays, and results are copied
+ * into caller-owned buffers.
+ *
+ * Conventions: every call returns 0 on success and a negative code on failure;
+ * ffip_last_error() then holds a thread-local message.  The library owns all
+ * device memory; the caller owns every host buffer it passes.  A handle owns one
+ * HIP stream and must not be used from two threads at once.  The GPU is chosen
+ * with HIP_VISIBLE_DEVICES; the reference's `np` / `decomposition` arguments have
+ * no counterpart (one GPU holds the whole grid).
+ */
+#ifndef FFIP_HIP_H
+#define FFIP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ffip_sim ffip_sim;
+
+#define FFIP_OK 0
+#define FFIP_EINVAL -1   /* bad argument, bad JSON or unsupported configuration */
+#define FFIP_ESTATE -2   /* call not valid in the handle's current state */
+#define FFIP_EDEVICE -3  /* HIP runtime error */
+
+/* Message of the last failed call on this thread. */
+const char *ffip_last_error(void);
+
+/* Parse a configuration (replaces reading ./config.json, app/run_sim_json.cpp:12).
+ * An extra key "precision": "fp64" | "fp32" selects the field storage type. */
+int ffip_sim_create(const char *config_json, ffip_sim **out);
+
+/* Provide an input dataset by name (replaces the input HDF5 file's datasets that
+ * the JSON references: "two medium box <sfx>" densities, ffip/geom.py:432-433;
+ * "inhom source amplitude/phase <sfx>", ffip/source.py:127-129). Data is copied. */
+int ffip_sim_set_dataset(ffip_sim *sim, const char *name, const double *data, size_t n);
+
+/* Record a time trace of one E/H component at a physical position, sampled after
+ * every full step with Simulation::at semantics (src/simulation.cpp:994-997,
+ * include/utility.hpp:983-1022).  Must precede ffip_sim_finalize_setup. */
+int ffip_sim_add_probe(ffip_sim *sim, double x, double y, double z, const char *component);
+
+/* Build the grid, materials, PML, sources and DFT monitors and upload them to
+ * the GPU (the set-up half of Simulation::init, src/simulation.cpp:127-188). */
+int ffip_sim_finalize_setup(ffip_sim *sim);
+
+/* Run a stop condition (Simulation::run, src/simulation.cpp:903-958):
+ * {"type":"time","time":T} | {"type":"decay",...} | {"type":"dft",...}.
+ * Steps continue from the current time step. */
+int ffip_sim_run(ffip_sim *sim, const char *stop_condition_json, int64_t *steps_taken);
+
+/* Advance exactly nsteps full steps (step_m then step_e, src/simulation.cpp:748-766). */
+int ffip_sim_step(ffip_sim *sim, int64_t nsteps);
+int ffip_sim_time_step(const ffip_sim *sim, int64_t *out);
+
+/* DFT outputs, one per "fields output" entry in the order given
+ * (Volume_Fields_DFT::output -> DFT_Hub::output_fields_collective, src/dft.cpp:273-320).
+ * dims = {nf, nz, ny, nx}; re/im are [nf][nz][ny][nx]; x/y/z/f are the 1-D axes. */
+int ffip_sim_dft_count(const ffip_sim *sim, int *out);
+int ffip_sim_dft_dims(const ffip_sim *sim, int k, int64_t dims[4]);
+int ffip_sim_dft_group(const ffip_sim *sim, int k, char *buf, size_t cap);
+int ffip_sim_get_dft(ffip_sim *sim, int k, double *re, double *im, double *x, double *y, double *z, double *f);
+
+/* Probe traces: capacity in steps must be set before stepping; out is [steps][nprobes]. */
+int ffip_sim_set_trace_capacity(ffip_sim *sim, int64_t steps);
+int ffip_sim_get_trace(ffip_sim *sim, double *out, int64_t cap_steps, int64_t *nsteps, int *nprobes);
+
+/* Simulation::at (src/simulation.cpp:994-997) for an E or H component. */
+int ffip_sim_probe(ffip_sim *sim, double x, double y, double z, const char *component, double *out);
+
+/* Raw values of one E/H component on its lattice inside the grid, [nz][ny][nx]
+ * (the points Yee_Iterator(grid_p1, grid_p2, ctype) visits, src/utility.cpp:28-125). */
+int ffip_sim_component_dims(const ffip_sim *sim, const char *component, int64_t dims[3]);
+int ffip_sim_get_component(ffip_sim *sim, const char *component, double *out);
+
+/* Geometry info request k (Simulation::geometry_info_request, src/simulation.cpp:190-233):
+ * coordinates of the points of a component inside geometry "geometry index". */
+int ffip_sim_geom_info_count(const ffip_sim *sim, int k, int64_t *npts);
+int ffip_sim_get_geom_info(const ffip_sim *sim, int k, double *x, double *y, double *z);
+
+/* Step nsteps with HIP events on the handle's stream: out[0] = elapsed ms of the
+ * whole region, out[1] / out[2] = summed ms of the electric / magnetic half-step
+ * kernels (only when per_kernel != 0).  No reference counterpart: measurement. */
+int ffip_sim_step_timed(ffip_sim *sim, int64_t nsteps, int per_kernel, double out[3]);
+
+/* Grid facts: out[0..2] = Yee cells per axis, out[3] = device bytes held,
+ * out[4] = dt, out[5] = dx. */
+int ffip_sim_info(const ffip_sim *sim, double out[6]);
+
+void ffip_sim_destroy(ffip_sim *sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFIP_HIP_H */
