@@ -1,0 +1,135 @@
+#!/usr/bin/env python3
+"""Throughput of the fused FDTD step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 4000] [--warmup 20] [--workload dispersive256]
+
+A step is one full FDTD time step (magnetic then electric half step, each one
+fused kernel) over the whole grid, with inputs resident in HBM.  The default
+workload is the dispersive-scattering configuration: 256^3 Yee cells, 12-cell
+CPML, a planar Gaussian current sheet, 32 seeded spheres (16 two-pole Lorentz,
+16 Drude), two DFT planes x 16 frequencies, fields and polarizations stored in
+FP32 with FP64 arithmetic, coefficients and DFT sums.
+
+Prints one JSON line.  `roofline` prices the dominant kernel's algorithmic bytes
+(ffip_sim_traffic) against its HIP-event duration over the timed region;
+`cpu_baseline` times the CPU oracle (the FP64 restatement of the reference's
+hot loop) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def workload(name):
+    from ffip_amd import workloads as W
+    if name == "dispersive256":
+        return W.dispersive_spheres(n=256, precision="fp32"), None, (
+            "dispersive scattering 256^3: 12-cell CPML, Ez Gaussian1 sheet, 32 seeded spheres "
+            "(16 two-pole Lorentz, 16 Drude), 2 Ez DFT planes x 16 freqs")
+    if name == "dispersive256-fp64":
+        return W.dispersive_spheres(n=256, precision="fp64"), None, "dispersive scattering 256^3, FP64 storage"
+    if name == "vacuum64":
+        return W.vacuum_dipole(), None, "vacuum Ez dipole 64^3, 8-cell CPML, 1 Ez DFT plane x 3 freqs"
+    raise SystemExit("unknown workload %r" % name)
+
+
+def cpu_baseline(cfg, ds, steps, cells):
+    """Time the oracle's C hot loop (single thread, FP64) for `steps` steps after
+    its set-up, on the same configuration."""
+    from oracle.oracle import OracleSim
+    cfg = dict(cfg)
+    cfg["precision"] = "fp64"
+    sim = OracleSim(cfg, ds, strict_pool=False)
+    t0 = time.perf_counter()
+    sim.run_steps(steps)
+    dt = time.perf_counter() - t0
+    return cells * steps / dt / 1e6
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="dispersive256")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the engine holds the whole grid on a single MI355X and has no "
+              "multi-GPU path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from ffip_amd._lib import LibSim
+
+    cfg, ds, desc = workload(args.workload)
+    cells = 1
+    for s in cfg["size"]:
+        cells *= int(round(s / (1 / cfg["resolution"])))
+    t_setup = time.perf_counter()
+    sim = LibSim(cfg, ds)
+    t_setup = time.perf_counter() - t_setup
+    print("[bench] set-up %.1f s, device memory %.2f GB" % (t_setup, sim.info()["device_bytes"] / 1e9),
+          file=sys.stderr, flush=True)
+    sim.step(args.warmup)
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+    t0 = time.perf_counter()
+    tm = sim.step_timed(args.steps, per_kernel=True)
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000 / args.steps
+    value = cells * args.steps / wall / 1e6
+    print("[bench] %d steps: wall %.3f s, events %.3f ms total, E kernels %.3f ms, H kernels %.3f ms"
+          % (args.steps, wall, tm["total_ms"], tm["e_ms"], tm["h_ms"]), file=sys.stderr, flush=True)
+
+    te, th = sim.traffic(0), sim.traffic(1)
+    e_avg = tm["e_ms"] / args.steps / 1000
+    h_avg = tm["h_ms"] / args.steps / 1000
+    dom_e = e_avg >= h_avg
+    tr, avg = (te, e_avg) if dom_e else (th, h_avg)
+    achieved = tr["total"] / avg / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_half<float,E>" if dom_e else "k_half<float,H>",
+            "avg_launch_us": round(avg * 1e6, 2), "bytes_per_launch": tr["total"]}
+    step_bytes = te["total"] + th["total"]
+    detail = {"electric": {"avg_us": round(e_avg * 1e6, 2), "traffic": te},
+              "magnetic": {"avg_us": round(h_avg * 1e6, 2), "traffic": th},
+              "step_hbm_frac": round(step_bytes / (ms_per_step / 1000) / 1e9 / HBM_PEAK_GBS, 4),
+              "setup_s": round(t_setup, 2)}
+    sim.close()
+
+    cpu = None
+    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        v = cpu_baseline(cfg, ds, args.cpu_steps, cells)
+        cpu = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+               "sample": "same workload and grid, %d steps after set-up, FP64 oracle (C restatement of the "
+                         "reference hot loop), 1 thread on %s" % (args.cpu_steps, platform.processor() or
+                                                                   platform.machine())}
+
+    line = {"metric": "Mcell-updates/s", "value": round(value, 1), "unit": "Mcell-updates/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": desc, "grid": [int(s) for s in cfg["size"]], "storage": cfg.get("precision"),
+                       "cells": cells},
+            "roofline": roof, "cpu_baseline": cpu, "detail": detail}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

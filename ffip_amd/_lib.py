@@ -49,6 +49,9 @@ _SIGS = {
                                ctypes.c_int),
     "ffip_sim_step_timed": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "ffip_sim_info": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sim_traffic": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sim_plan": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)],
+                      ctypes.c_int),
     "ffip_sim_destroy": ([ctypes.c_void_p], None),
 }
 
@@ -77,6 +80,27 @@ def _check(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def plan(config, datasets: Optional[Dict[str, np.ndarray]] = None, probes: Sequence = ()):
+    """Host-only set-up summary (ffip_sim_plan); needs no GPU."""
+    L = lib()
+    text = config if isinstance(config, str) else json.dumps(config)
+    h = ctypes.c_void_p()
+    _check(L.ffip_sim_create(text.encode(), ctypes.byref(h)))
+    try:
+        for name, arr in (datasets or {}).items():
+            a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel())
+            _check(L.ffip_sim_set_dataset(h, name.encode(), _ptr(a), a.size))
+        for pos, comp in probes:
+            _check(L.ffip_sim_add_probe(h, float(pos[0]), float(pos[1]), float(pos[2]), comp.encode()))
+        need = ctypes.c_size_t()
+        _check(L.ffip_sim_plan(h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value)
+        _check(L.ffip_sim_plan(h, buf, need.value, None))
+        return json.loads(buf.value.decode())
+    finally:
+        L.ffip_sim_destroy(h)
 
 
 class LibSim:
@@ -141,6 +165,12 @@ class LibSim:
         out = np.zeros(6)
         _check(lib().ffip_sim_info(self._h, _ptr(out)))
         return dict(cells=(out[0], out[1], out[2]), device_bytes=out[3], dt=out[4], dx=out[5])
+
+    def traffic(self, half: int):
+        out = np.zeros(7)
+        _check(lib().ffip_sim_traffic(self._h, int(half), _ptr(out)))
+        keys = ("fields", "material", "polarization", "cpml", "dft", "total", "points")
+        return dict(zip(keys, out.tolist()))
 
     # -- results
     def dft_count(self) -> int:

@@ -1,4 +1,6 @@
 // extern "C" boundary of libffip_hip.so (declared in include/ffip_hip.h).
+#include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -284,6 +286,220 @@ int ffip_sim_step_timed(ffip_sim* sim, int64_t nsteps, int per_kernel, double ou
         out[1] = t.e_ms;
         out[2] = t.h_ms;
     });
+}
+
+namespace {
+
+struct JsonOut {
+    std::string s;
+    void num(double v)
+    {
+        char b[40];
+        std::snprintf(b, sizeof b, "%.17g", v);
+        s += b;
+    }
+    void inum(int64_t v) { s += std::to_string(v); }
+    void key(const char* k)
+    {
+        s += '"';
+        s += k;
+        s += "\":";
+    }
+};
+
+std::string plan_json(const Setup& S)
+{
+    JsonOut o;
+    const Layout& L = S.L;
+    o.s += "{";
+    o.key("P1");
+    o.s += "[" + std::to_string(L.P1[0]) + "," + std::to_string(L.P1[1]) + "," + std::to_string(L.P1[2]) + "],";
+    o.key("P2");
+    o.s += "[" + std::to_string(L.P2[0]) + "," + std::to_string(L.P2[1]) + "," + std::to_string(L.P2[2]) + "],";
+    o.key("M");
+    o.s += "[" + std::to_string(L.M[0]) + "," + std::to_string(L.M[1]) + "," + std::to_string(L.M[2]) + "],";
+    o.key("n");
+    o.s += "[" + std::to_string(L.n[0]) + "," + std::to_string(L.n[1]) + "," + std::to_string(L.n[2]) + "],";
+    o.key("lo");
+    o.s += "[";
+    for (int c = 0; c < 6; ++c)
+        o.s += std::string(c ? "," : "") + "[" + std::to_string(L.lo[c][0]) + "," + std::to_string(L.lo[c][1]) + "," +
+               std::to_string(L.lo[c][2]) + "]";
+    o.s += "],";
+    o.key("hi");
+    o.s += "[";
+    for (int c = 0; c < 6; ++c)
+        o.s += std::string(c ? "," : "") + "[" + std::to_string(L.hi[c][0]) + "," + std::to_string(L.hi[c][1]) + "," +
+               std::to_string(L.hi[c][2]) + "]";
+    o.s += "],";
+    o.key("dt");
+    o.num(S.dt);
+    o.s += ",";
+    o.key("slab");  // [axis][parity] -> list of indices with c != 0, and their (b, c, ik)
+    o.s += "[";
+    for (int a = 0; a < 3; ++a) {
+        o.s += a ? ",[" : "[";
+        for (int p = 0; p < 2; ++p) {
+            o.s += p ? ",[" : "[";
+            bool first = true;
+            for (int q = 0; q < L.n[a]; ++q)
+                if (S.pml[a].smap[p][q] >= 0) {
+                    o.s += first ? "[" : ",[";
+                    first = false;
+                    o.inum(q);
+                    o.s += ",";
+                    o.num(S.pml[a].b[p][q]);
+                    o.s += ",";
+                    o.num(S.pml[a].c[p][q]);
+                    o.s += ",";
+                    o.num(S.pml[a].ik[p][q]);
+                    o.s += "]";
+                }
+            o.s += "]";
+        }
+        o.s += "]";
+    }
+    o.s += "],";
+    o.key("dipoles");
+    o.s += "[";
+    for (int h = 0; h < 2; ++h)
+        for (size_t q = 0; q < S.dip[h].size(); ++q) {
+            const Dipole& d = S.dip[h][q];
+            o.s += (h == 0 && q == 0) || (h == 1 && q == 0 && S.dip[0].empty()) ? "[" : ",[";
+            o.inum(d.comp);
+            o.s += ",";
+            o.inum(d.i);
+            o.s += ",";
+            o.inum(d.j);
+            o.s += ",";
+            o.inum(d.k);
+            o.s += ",";
+            o.num(d.amp);
+            o.s += ",";
+            o.num(d.start);
+            o.s += ",";
+            o.num(d.end);
+            o.s += ",";
+            o.num(d.width);
+            o.s += "]";
+        }
+    o.s += "],";
+    o.key("units");
+    o.s += "[";
+    bool first = true;
+    for (int h = 0; h < 2; ++h)
+        for (auto& U : S.units[h]) {
+            o.s += first ? "{" : ",{";
+            first = false;
+            o.key("comp");
+            o.inum(U.comp);
+            o.s += ",";
+            o.key("has_local");
+            o.inum(U.has_local ? 1 : 0);
+            o.s += ",";
+            o.key("box");
+            o.s += "[" + std::to_string(U.box[0][0]) + "," + std::to_string(U.box[0][1]) + "," +
+                   std::to_string(U.box[1][0]) + "," + std::to_string(U.box[1][1]) + "," +
+                   std::to_string(U.box[2][0]) + "," + std::to_string(U.box[2][1]) + "],";
+            o.key("nf");
+            o.inum(int64_t(U.freqs.size()));
+            o.s += "}";
+        }
+    o.s += "],";
+    o.key("materials");  // per half: uniform flag, table (g_inf, npoles), per-component id histogram
+    o.s += "[";
+    for (int h = 0; h < 2; ++h) {
+        const HalfMaterial& M = S.mat[h];
+        o.s += h ? ",{" : "{";
+        o.key("uniform");
+        o.inum(M.uniform ? 1 : 0);
+        o.s += ",";
+        o.key("table");
+        o.s += "[";
+        for (size_t e = 0; e < M.table.eps.size(); ++e) {
+            o.s += e ? ",[" : "[";
+            o.num(M.table.eps[e]);
+            o.s += ",";
+            o.inum(M.table.npoles[e]);
+            o.s += "]";
+        }
+        o.s += "],";
+        o.key("hist");
+        o.s += "[";
+        for (int c = 0; c < 3; ++c) {
+            std::vector<int64_t> hist(256, 0);
+            const int comp = h * 3 + c;
+            if (!M.uniform)
+                for (int k = L.lo[comp][2]; k <= L.hi[comp][2]; ++k)
+                    for (int j = L.lo[comp][1]; j <= L.hi[comp][1]; ++j)
+                        for (int i = L.lo[comp][0]; i <= L.hi[comp][0]; ++i)
+                            ++hist[M.ids[c][L.index(i, j, k)]];
+            o.s += c ? ",[" : "[";
+            for (int e = 0; e < 256; ++e) {
+                if (e)
+                    o.s += ",";
+                o.inum(hist[e]);
+            }
+            o.s += "]";
+        }
+        o.s += "],";
+        o.key("dense_points");
+        o.s += "[";
+        for (int c = 0; c < 3; ++c) {
+            if (c)
+                o.s += ",";
+            o.inum(M.dense[c].present ? M.dense[c].npts() : 0);
+        }
+        o.s += "]}";
+    }
+    o.s += "],";
+    o.key("faces");
+    o.inum(int64_t(S.faces[0].size() + S.faces[1].size()));
+    o.s += ",";
+    o.key("probe_terms");
+    o.s += "[";
+    for (size_t p = 0; p < S.probes.size(); ++p) {
+        o.s += p ? ",[" : "[";
+        for (size_t t = 0; t < S.probes[p].size(); ++t) {
+            o.s += t ? ",[" : "[";
+            o.inum(S.probes[p][t].comp);
+            o.s += ",";
+            o.inum(S.probes[p][t].index);
+            o.s += ",";
+            o.num(S.probes[p][t].w);
+            o.s += "]";
+        }
+        o.s += "]";
+    }
+    o.s += "]}";
+    return o.s;
+}
+
+}  // namespace
+
+int ffip_sim_plan(const ffip_sim* sim, char* buf, size_t cap, size_t* needed)
+{
+    if (!sim)
+        return fail(FFIP_EINVAL, "null handle");
+    return guard([&] {
+        Setup S;
+        build_setup(sim->config, sim->datasets, sim->probes, S);
+        std::string j = plan_json(S);
+        if (needed)
+            *needed = j.size() + 1;
+        if (buf && cap) {
+            size_t n = std::min(cap - 1, j.size());
+            std::memcpy(buf, j.data(), n);
+            buf[n] = 0;
+        }
+    });
+}
+
+int ffip_sim_traffic(const ffip_sim* sim, int half, double out[7])
+{
+    if (!out)
+        return fail(FFIP_EINVAL, "null argument");
+    return guard([&] { need_engine(const_cast<ffip_sim*>(sim)).traffic(half, out); });
 }
 
 int ffip_sim_info(const ffip_sim* sim, double out[6])

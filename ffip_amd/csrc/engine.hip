@@ -462,6 +462,7 @@ struct EngineImpl {
     double* pr_w = nullptr;
     double* trace = nullptr;
     int64_t trace_cap = 0;
+    double traffic[2][7] = {};
 
     template <typename X>
     X* dalloc(size_t count, bool zero = true)
@@ -710,6 +711,51 @@ void EngineImpl::build()
         if (faces[h].size() > size_t(MAXF))
             throw std::runtime_error("too many boundary faces");
     }
+    // algorithmic traffic per half-step launch
+    for (int h = 0; h < 2; ++h) {
+        const HalfMaterial& M = s.mat[h];
+        double nreal = 0, nsrc = 0, ade = 0, psi = 0, dftb = 0;
+        for (int c = 0; c < 3; ++c) {
+            nreal += double(L.count(h * 3 + c));
+            nsrc += double(L.count((1 - h) * 3 + c));
+        }
+        if (!M.uniform)
+            for (int c = 0; c < 3; ++c) {
+                const int comp = h * 3 + c;
+                for (int k = L.lo[comp][2]; k <= L.hi[comp][2]; ++k)
+                    for (int j = L.lo[comp][1]; j <= L.hi[comp][1]; ++j)
+                        for (int i = L.lo[comp][0]; i <= L.hi[comp][0]; ++i) {
+                            int id = M.ids[c][L.index(i, j, k)];
+                            int np = id == 255 ? M.dense[c].npoles : M.table.npoles[id];
+                            ade += 3.0 * np;  // read P, read P-, write P+
+                        }
+            }
+        const int par = h == 0 ? 0 : 1;
+        for (int c = 0; c < 3; ++c) {
+            const int comp = h * 3 + c;
+            for (int a : {(c + 1) % 3, (c + 2) % 3}) {
+                const std::vector<int>& sm = s.pml[a].smap[par];
+                double inslab = 0;
+                for (int q = L.lo[comp][a]; q <= L.hi[comp][a]; ++q)
+                    inslab += sm[q] >= 0 ? 1 : 0;
+                double other = 1;
+                for (int b = 0; b < 3; ++b)
+                    if (b != a)
+                        other *= std::max(0, L.hi[comp][b] - L.lo[comp][b] + 1);
+                psi += 2.0 * inslab * other;  // read + write psi
+            }
+        }
+        for (auto& U : units[h])
+            dftb += double(U.npts) * U.nf * 32.0;  // re, im: read + write, FP64
+        double* t = traffic[h];
+        t[0] = (2 * nreal + nsrc) * double(tsz);
+        t[1] = M.uniform ? 0.0 : nreal;
+        t[2] = ade * double(tsz);
+        t[3] = psi * double(tsz);
+        t[4] = dftb;
+        t[5] = t[0] + t[1] + t[2] + t[3] + t[4];
+        t[6] = nreal;
+    }
     // probes
     nprobes = int(s.probes.size());
     if (nprobes) {
@@ -939,6 +985,14 @@ const Setup& Engine::setup() const { return *p_->S; }
 int64_t Engine::time_step() const { return p_->n; }
 
 void Engine::sync() { HIP_OK(hipStreamSynchronize(p_->stream)); }
+
+void Engine::traffic(int h, double out[7]) const
+{
+    if (h < 0 || h > 1)
+        throw std::runtime_error("half must be 0 (electric) or 1 (magnetic)");
+    for (int q = 0; q < 7; ++q)
+        out[q] = p_->traffic[h][q];
+}
 double Engine::device_bytes() const { return p_->bytes; }
 
 void Engine::step(int64_t nsteps)

@@ -53,6 +53,11 @@ public:
     void get_component(int ct, double* out);
     void sync();
     double device_bytes() const;
+    // Algorithmic HBM bytes of one launch of the half-step kernel (h = 0 electric,
+    // 1 magnetic): out = {fields, material ids, polarization, CPML psi, DFT sums,
+    // total, points updated}.  Each array element the half step must read or write
+    // counts once; neighbour re-reads and table lookups count zero.
+    void traffic(int h, double out[7]) const;
 
 private:
     std::unique_ptr<EngineImpl> p_;

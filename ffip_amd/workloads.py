@@ -146,7 +146,16 @@ def blurred_density(shape, seed=3, radius=3, sigma=1.5):
     k = np.exp(-0.5 * (t / sigma) ** 2)
     k /= k.sum()
     for ax in range(3):
-        rho = np.apply_along_axis(lambda v: np.convolve(v, k, mode="same"), ax, rho)
+        n = rho.shape[ax]
+        pad = [(0, 0)] * 3
+        pad[ax] = (radius, radius)
+        p = np.pad(rho, pad)
+        out = np.zeros_like(rho)
+        for q, wq in enumerate(k):
+            sl = [slice(None)] * 3
+            sl[ax] = slice(q, q + n)
+            out += wq * p[tuple(sl)]
+        rho = out
     return rho
 
 

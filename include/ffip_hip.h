@@ -95,6 +95,18 @@ int ffip_sim_get_geom_info(const ffip_sim *sim, int k, double *x, double *y, dou
  * kernels (only when per_kernel != 0).  No reference counterpart: measurement. */
 int ffip_sim_step_timed(ffip_sim *sim, int64_t nsteps, int per_kernel, double out[3]);
 
+/* Host-only dry run of the set-up (no GPU needed): writes a JSON summary of the
+ * grid layout, PML slabs, dipole list, DFT units and material table into buf
+ * (cap bytes); *needed receives the full length + 1.  Probes added before the
+ * call are included.  No reference counterpart: diagnostics and tests. */
+int ffip_sim_plan(const ffip_sim *sim, char *buf, size_t cap, size_t *needed);
+
+/* Algorithmic HBM bytes of one launch of the half-step kernel (half 0 =
+ * electric, 1 = magnetic): out = {fields, material ids, polarization state,
+ * CPML auxiliaries, DFT sums, total, points updated}.  No reference counterpart:
+ * measurement. */
+int ffip_sim_traffic(const ffip_sim *sim, int half, double out[7]);
+
 /* Grid facts: out[0..2] = Yee cells per axis, out[3] = device bytes held,
  * out[4] = dt, out[5] = dx. */
 int ffip_sim_info(const ffip_sim *sim, double out[6]);

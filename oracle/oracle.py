@@ -316,6 +316,9 @@ class Box:
         self.hi = [max(self.p1[a], self.p2[a]) for a in range(3)]
         self.medium = medium
 
+    def bounds(self):
+        return self.lo, self.hi
+
     def inside(self, X):
         m = np.ones(X.shape[0], dtype=bool)
         for a in range(3):
@@ -336,6 +339,9 @@ class Sphere(Box):
         self.c = [float(v) for v in center]
         self.r2 = radius * radius
         self.medium = medium
+        rr = abs(radius)
+        self.lo = [v - rr - 1 for v in self.c]
+        self.hi = [v + rr + 1 for v in self.c]
 
     def inside(self, X):
         d2 = ((X[:, 0] - self.c[0]) ** 2 + (X[:, 1] - self.c[1]) ** 2 + (X[:, 2] - self.c[2]) ** 2)
@@ -460,8 +466,9 @@ class OracleSim:
                     full step with ``Simulation::at`` semantics.
     """
 
-    def __init__(self, config, datasets=None, probes=(), trace_steps=0):
+    def __init__(self, config, datasets=None, probes=(), trace_steps=0, strict_pool=True):
         self.cfg = config
+        self.strict_pool = strict_pool
         self.datasets = datasets or {}
         self._keep = []
         sc = float(config.get("courant number", 0.5))
@@ -533,7 +540,7 @@ class OracleSim:
             for s in m.m_sus:
                 if s not in self.m_pool:
                     self.m_pool.append(s)
-        if len(self.e_pool) > 31 or len(self.m_pool) > 31:
+        if self.strict_pool and (len(self.e_pool) > 31 or len(self.m_pool) > 31):
             raise ValueError("more than 31 distinct susceptibilities (src/structure.cpp:261)")
 
     # -- src/structure.cpp:126-153
@@ -652,42 +659,64 @@ class OracleSim:
             ct = CTYPE[name]
             half = 0 if is_e_point(ct) else 1
             p1, p2 = component_interior(self.grid.grid_p1, self.grid.grid_p2, ct)
-            X = yee_coords(p1, p2, 2)
-            n = X.shape[0]
-            eps = np.full(n, bg.eps)
-            mu = np.full(n, bg.mu)
-            ea = np.tile(bg.e_amp, (n, 1))
-            ma = np.tile(bg.m_amp, (n, 1))
-            gidx = np.full(n, -1, dtype=np.int64)
-            taken = np.zeros(n, dtype=bool)
-            Xf = X.astype(float)
+            dims = [(p2[a] - p1[a]) // 2 + 1 for a in range(3)]
+            if min(dims) <= 0:
+                continue
+            # first geometry (list order) strictly containing each lattice point,
+            # tested only inside each geometry's bounding box
+            gidx = np.full((dims[2], dims[1], dims[0]), -1, dtype=np.int32)
             for gi, g in enumerate(glist):
-                cand = ~taken
-                if not cand.any():
-                    break
-                sel = np.zeros(n, dtype=bool)
-                sel[cand] = g.inside(Xf[cand])
-                if sel.any():
-                    e_, m_, ea_, ma_ = g.media(Xf[sel])
-                    eps[sel], mu[sel], ea[sel], ma[sel] = e_, m_, ea_, ma_
-                    gidx[sel] = gi
-                    taken |= sel
-            self.geom_index[name] = (X, gidx)
-            idx = self.grid.index(X[:, 0], X[:, 1], X[:, 2])
-            amps = ea if half == 0 else ma
-            ginf = eps if half == 0 else mu
+                lo, hi = g.bounds()
+                r0 = [max(0, int(math.floor((lo[a] - p1[a]) / 2))) for a in range(3)]
+                r1 = [min(dims[a] - 1, int(math.ceil((hi[a] - p1[a]) / 2))) for a in range(3)]
+                if any(r0[a] > r1[a] for a in range(3)):
+                    continue
+                sub = gidx[r0[2]:r1[2] + 1, r0[1]:r1[1] + 1, r0[0]:r1[0] + 1]
+                zz = p1[2] + 2 * np.arange(r0[2], r1[2] + 1)
+                yy = p1[1] + 2 * np.arange(r0[1], r1[1] + 1)
+                xx = p1[0] + 2 * np.arange(r0[0], r1[0] + 1)
+                Z, Y, X = np.meshgrid(zz, yy, xx, indexing="ij")
+                P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1).astype(float)
+                free = sub.ravel() == -1
+                ins = np.zeros(P.shape[0], dtype=bool)
+                ins[free] = g.inside(P[free])
+                flat = sub.reshape(-1).copy()
+                flat[ins] = gi
+                sub[...] = flat.reshape(sub.shape)
+            gflat = gidx.ravel()
+            self.geom_index[name] = (p1, dims, gidx)
+            Xall = yee_coords(p1, p2, 2)
+            idx = self.grid.index(Xall[:, 0], Xall[:, 1], Xall[:, 2])
             coefs = e_c if half == 0 else m_c
-            nz = np.abs(amps) > 1e-4  # get_non_zeros_from_array, src/structure.cpp:252-260
-            mask = np.zeros(n, dtype=np.int64)
-            for i in range(amps.shape[1]):
-                mask |= nz[:, i].astype(np.int64) << i
-            for mv in np.unique(mask):
-                sel = mask == mv
-                poles = [i for i in range(amps.shape[1]) if (int(mv) >> i) & 1]
-                grp = self.groups[half].setdefault(int(mv), dict(poles=poles, idx=[], ginf=[], amp=[]))
-                grp["idx"].append(idx[sel])
-                grp["ginf"].append(ginf[sel])
-                grp["amp"].append(amps[sel][:, poles] if poles else np.zeros((int(sel.sum()), 0)))
+            npool = len(self.e_pool) if half == 0 else len(self.m_pool)
+
+            def add(sel_idx, ginf, amps):
+                nz = np.abs(amps) > 1e-4  # get_non_zeros_from_array, src/structure.cpp:252-260
+                mask = np.zeros(amps.shape[0], dtype=np.int64)
+                for i in range(npool):
+                    mask |= nz[:, i].astype(np.int64) << i
+                for mv in np.unique(mask):
+                    sel = mask == mv
+                    poles = [i for i in range(npool) if (int(mv) >> i) & 1]
+                    grp = self.groups[half].setdefault(int(mv), dict(poles=poles, idx=[], ginf=[], amp=[]))
+                    grp["idx"].append(sel_idx[sel])
+                    grp["ginf"].append(ginf[sel])
+                    grp["amp"].append(amps[sel][:, poles] if poles else np.zeros((int(sel.sum()), 0)))
+
+            for src_g in [-1] + list(range(len(glist))):
+                sel = gflat == src_g
+                m = int(sel.sum())
+                if m == 0:
+                    continue
+                g = glist[src_g] if src_g >= 0 else None
+                if isinstance(g, TwoMediumBox):
+                    eps, mu, ea, ma = g.media(Xall[sel].astype(float))
+                else:
+                    med = bg if g is None else g.medium
+                    eps, mu = np.full(m, med.eps), np.full(m, med.mu)
+                    ea = np.tile(med.e_amp, (m, 1))
+                    ma = np.tile(med.m_amp, (m, 1))
+                add(idx[sel], eps if half == 0 else mu, ea if half == 0 else ma)
         for half in (0, 1):
             coefs = e_c if half == 0 else m_c
             for mv, grp in self.groups[half].items():
