@@ -102,10 +102,18 @@ def main():
     dom_e = e_avg >= h_avg
     tr, avg = (te, e_avg) if dom_e else (th, h_avg)
     achieved = tr["total"] / avg / 1e9
+    kname = ("k_half<%s,%s>" % ("float" if cfg.get("precision") == "fp32" else "double", "E" if dom_e else "H"))
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+    # workload (calibrated against a plain copy), when a measurement is on file
+    traffic = None
+    try:
+        with open(os.path.join(HERE, "profiles", "traffic.json")) as f:
+            traffic = json.load(f).get(args.workload, {}).get(kname)
+    except (OSError, ValueError):
+        traffic = None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_half<float,E>" if dom_e else "k_half<float,H>",
-            "avg_launch_us": round(avg * 1e6, 2), "bytes_per_launch": tr["total"]}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": kname, "avg_launch_us": round(avg * 1e6, 2), "bytes_per_launch": tr["total"]}
     step_bytes = te["total"] + th["total"]
     detail = {"electric": {"avg_us": round(e_avg * 1e6, 2), "traffic": te},
               "magnetic": {"avg_us": round(h_avg * 1e6, 2), "traffic": th},

@@ -52,6 +52,8 @@ _SIGS = {
     "ffip_sim_traffic": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "ffip_sim_plan": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)],
                       ctypes.c_int),
+    "ffip_transpose": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                        ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
     "ffip_sim_destroy": ([ctypes.c_void_p], None),
 }
 
@@ -80,6 +82,25 @@ def _check(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def transpose(gx, gy, gz, pt):
+    """Adjoint of zero-padded trilinear interpolation onto the grid (gx, gy, gz):
+    pt[:, :3] are positions, pt[:, 3] values; returns [nz, ny, nx].  Same
+    contract as the reference's _ffip.transpose (app/ffip.cpp:308-352): KeyError
+    for bad grid arrays, ValueError when pt is not (n, 4)."""
+    gx, gy, gz = (np.ascontiguousarray(np.asarray(g, dtype=np.float64)) for g in (gx, gy, gz))
+    pt = np.ascontiguousarray(np.asarray(pt, dtype=np.float64))
+    if gx.ndim != 1 or gy.ndim != 1 or gz.ndim != 1 or pt.ndim != 2:
+        raise KeyError("some dimensions are not right")
+    if gx.size < 1 or gy.size < 1 or gz.size < 1:
+        raise KeyError("some grid size is 0")
+    if pt.shape[1] != 4:
+        raise ValueError("shape(1) of pts is not 4")
+    out = np.zeros((gz.size, gy.size, gx.size))
+    _check(lib().ffip_transpose(_ptr(gx), gx.size, _ptr(gy), gy.size, _ptr(gz), gz.size, _ptr(pt), pt.shape[0],
+                                _ptr(out)))
+    return out
 
 
 def plan(config, datasets: Optional[Dict[str, np.ndarray]] = None, probes: Sequence = ()):

@@ -519,4 +519,59 @@ int ffip_sim_info(const ffip_sim* sim, double out[6])
 
 void ffip_sim_destroy(ffip_sim* sim) { delete sim; }
 
+// interpn<N>::transpose with zero padding (include/utility.hpp:702-742, 797-828),
+// driven as in app/ffip.cpp:308-352
+static void transpose_axis(double* data, const int64_t* dims, const size_t* strides, const double* q, int nd,
+                           double val)
+{
+    const int64_t n = dims[0];
+    auto rest = [&](double* ptr, double v) {
+        if (nd == 1)
+            *ptr += v;
+        else
+            transpose_axis(ptr, dims + 1, strides + 1, q + 1, nd - 1, v);
+    };
+    const double xq = q[0];
+    if (n == 1) {
+        rest(data, val);
+        return;
+    }
+    if (xq <= 0) {
+        if (xq <= -1)
+            return;
+        rest(data, val * (1 + xq));
+        return;
+    }
+    if (xq >= double(n - 1)) {
+        if (xq >= double(n))
+            return;
+        rest(data + strides[0] * size_t(n - 1), val * (double(n) - xq));
+        return;
+    }
+    const size_t index = size_t(xq);
+    const double tx = xq - double(index);
+    rest(data + strides[0] * (index + 1), val * tx);
+    rest(data + strides[0] * index, val * (1 - tx));
+}
+
+int ffip_transpose(const double* gx, int64_t nx, const double* gy, int64_t ny, const double* gz, int64_t nz,
+                   const double* pts, int64_t npts, double* out)
+{
+    if (!gx || !gy || !gz || !out || (npts > 0 && !pts))
+        return fail(FFIP_EINVAL, "null argument");
+    if (nx < 1 || ny < 1 || nz < 1)
+        return fail(FFIP_EINVAL, "some grid size is 0");
+    return guard([&] {
+        double dx = nx > 1 ? gx[1] - gx[0] : 1, dy = ny > 1 ? gy[1] - gy[0] : 1, dz = nz > 1 ? gz[1] - gz[0] : 1;
+        std::fill(out, out + nx * ny * nz, 0.0);
+        const int64_t dims[3] = {nz, ny, nx};
+        const size_t strides[3] = {size_t(ny * nx), size_t(nx), 1};
+        for (int64_t i = 0; i < npts; ++i) {
+            const double* p = pts + 4 * i;
+            const double q[3] = {(p[2] - gz[0]) / dz, (p[1] - gy[0]) / dy, (p[0] - gx[0]) / dx};
+            transpose_axis(out, dims, strides, q, 3, p[3]);
+        }
+    });
+}
+
 }  // extern "C"

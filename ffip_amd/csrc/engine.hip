@@ -23,7 +23,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <limits>
 #include <sstream>
 #include <stdexcept>
 #include <unordered_map>
@@ -39,18 +41,19 @@ namespace ffip_amd {
             throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr); \
     } while (0)
 
-static constexpr int BX = 64;
-static constexpr int BY = 4;
-static constexpr int LZ = 32;
+static constexpr int BX = 32;
+static constexpr int BY = 8;
 static constexpr int MAXU = 16;
 static constexpr int MAXS = 8;  // polarization slots per point
+static constexpr int MAXMAT = 256;
 static constexpr int PH_CHUNK = 1024;
 
 struct DftDev {
-    int comp;     // 0..2 within the half
+    int comp;  // 0..2 within the half
     int box[3][2];
     int dims[3];
     int nf, mf0, is_db;
+    int fused;  // accumulated inside the half-step kernel (else by k_dft_sparse)
     int64_t npts;
     double* re;
     double* im;
@@ -66,26 +69,29 @@ struct PmlDev {
     int W;
 };
 
+struct MatEntry {
+    double inv, eps;
+    int np, p0;
+};
+
 template <typename T>
 struct HalfArgs {
     T* u[3];
     const T* s[3];
-    int64_t pitch, plane;
+    int pitch, plane;
     int n[3];
     int lo[3][3], hi[3][3];
     int dlo[3], dhi[3];
+    int gx, gy, gz, nblocks, lz;
     double inv_dx, dt;
     // materials
     double u_inv, u_eps;
-    const uint8_t* ids[3];
-    const double* m_inv;
-    const double* m_eps;
-    const int* m_np;
-    const int* m_p0;
+    const uint32_t* ids;  // per cell: id of component 0 | 1 << 8 | 2 << 16
+    const MatEntry* mats;
+    int nmats;
     const double* p_c1;
     const double* p_c2;
     const double* p_sc3;
-    int d_present[3];
     int d_box[3][3][2];
     int d_np[3];
     const double* d_inv[3];
@@ -109,61 +115,68 @@ __device__ __forceinline__ double gaussian1(double t, double width)
     return arg * exp(-arg * arg / 2);
 }
 
-// psi index of a point in the slab of axis a (slab position s)
-__device__ __forceinline__ int64_t psi_index(int a, int s, int i, int j, int k, const int* n, int W)
+// one CPML term (src/fields.cpp:89-109): psi <- b psi + c d; returns psi + d / kappa
+template <typename T>
+__device__ __forceinline__ double cpml_term(T* psi, int q, double b, double c, double ik, double d)
 {
-    if (a == 0)
-        return int64_t(s) + int64_t(W) * (j + int64_t(n[1]) * k);
-    if (a == 1)
-        return int64_t(i) + int64_t(n[0]) * (s + int64_t(W) * k);
-    return int64_t(i) + int64_t(n[0]) * (j + int64_t(n[1]) * s);
+    const double ps = b * double(psi[q]) + c * d;
+    psi[q] = T(ps);
+    return ps + d * ik;
 }
 
-template <typename T, int C>
-__device__ __forceinline__ double cpml(const HalfArgs<T>& A, int a, bool first, int i, int j, int k, double d)
-{
-    const PmlDev& P = A.pml[a];
-    if (P.W == 0)
-        return d;
-    const int coord = a == 0 ? i : (a == 1 ? j : k);
-    const int s = P.smap[coord];
-    if (s < 0)
-        return d;
-    T* psi = static_cast<T*>(first ? P.psi1 : P.psi2);
-    const int64_t q = psi_index(a, s, i, j, k, A.n, P.W);
-    const double ps = P.b[coord] * double(psi[q]) + P.c[coord] * d;
-    psi[q] = T(ps);
-    return ps + d * P.ik[coord];
-}
+struct PlaneCtx {
+    int i, j, k, o;
+    // slab positions and coefficients of the three axes at this point
+    int sx, sy, sz;
+    double bx, cx, ikx, by, cy, iky, bz, cz, ikz;
+};
 
 template <typename T, bool MAT, int C>
-__device__ __forceinline__ void update_point(const HalfArgs<T>& A, int i, int j, int k, int64_t idx, double d1,
-                                             double d2)
+__device__ __forceinline__ void update_point(const HalfArgs<T>& A, const MatEntry* smat, const PlaneCtx& P,
+                                             double d1, double d2, T e_in, uint32_t idword)
 {
-    constexpr int a1 = (C + 1) % 3;
-    constexpr int a2 = (C + 2) % 3;
-    const double t1 = cpml<T, C>(A, a1, true, i, j, k, d1);
-    const double t2 = cpml<T, C>(A, a2, false, i, j, k, d2);
+    // term 1 differentiates along a1 = (C+1)%3, term 2 along a2 = (C+2)%3
+    double t1 = d1, t2 = d2;
+    if (C == 0) {  // a1 = y, a2 = z
+        if (P.sy >= 0)
+            t1 = cpml_term<T>(static_cast<T*>(A.pml[1].psi1), P.i + A.n[0] * (P.sy + A.pml[1].W * P.k), P.by, P.cy,
+                              P.iky, d1);
+        if (P.sz >= 0)
+            t2 = cpml_term<T>(static_cast<T*>(A.pml[2].psi2), P.i + A.n[0] * (P.j + A.n[1] * P.sz), P.bz, P.cz,
+                              P.ikz, d2);
+    } else if (C == 1) {  // a1 = z, a2 = x
+        if (P.sz >= 0)
+            t1 = cpml_term<T>(static_cast<T*>(A.pml[2].psi1), P.i + A.n[0] * (P.j + A.n[1] * P.sz), P.bz, P.cz,
+                              P.ikz, d1);
+        if (P.sx >= 0)
+            t2 = cpml_term<T>(static_cast<T*>(A.pml[0].psi2), P.sx + A.pml[0].W * (P.j + A.n[1] * P.k), P.bx, P.cx,
+                              P.ikx, d2);
+    } else {  // a1 = x, a2 = y
+        if (P.sx >= 0)
+            t1 = cpml_term<T>(static_cast<T*>(A.pml[0].psi1), P.sx + A.pml[0].W * (P.j + A.n[1] * P.k), P.bx, P.cx,
+                              P.ikx, d1);
+        if (P.sy >= 0)
+            t2 = cpml_term<T>(static_cast<T*>(A.pml[1].psi2), P.i + A.n[0] * (P.sy + A.pml[1].W * P.k), P.by, P.cy,
+                              P.iky, d2);
+    }
     const double acc = t1 - t2;  // increment of D/dt (B/dt)
-    T* __restrict__ u = A.u[C];
-    const double e_old = double(u[idx]);
+    const double e_old = double(e_in);
     double e_new, eps, sumP = 0;
     if (!MAT) {
         e_new = e_old + A.dt * acc * A.u_inv;
         eps = A.u_eps;
     } else {
-        const int id = A.ids[C][idx];
+        const int id = int((idword >> (8 * C)) & 255u);
         double inv, dP = 0;
         if (id != 255) {
-            inv = A.m_inv[id];
-            eps = A.m_eps[id];
-            const int np = A.m_np[id];
-            const int p0 = A.m_p0[id];
-            for (int s = 0; s < np; ++s) {
-                const int64_t o = (int64_t(s) * 3 + C) * A.pstride + idx;
+            const MatEntry m = smat[id];
+            inv = m.inv;
+            eps = m.eps;
+            for (int s = 0; s < m.np; ++s) {
+                const int64_t o = (int64_t(s) * 3 + C) * A.pstride + P.o;
                 const double pc = double(A.Pcur[o]);
                 const double pp = double(A.Pprev[o]);
-                const double pn = A.p_c1[p0 + s] * pc + A.p_c2[p0 + s] * pp + A.p_sc3[p0 + s] * e_old;
+                const double pn = A.p_c1[m.p0 + s] * pc + A.p_c2[m.p0 + s] * pp + A.p_sc3[m.p0 + s] * e_old;
                 const T pt = T(pn);
                 A.Pprev[o] = pt;
                 dP += double(pt) - pc;
@@ -172,12 +185,12 @@ __device__ __forceinline__ void update_point(const HalfArgs<T>& A, int i, int j,
         } else {
             const int b00 = A.d_box[C][0][0], b10 = A.d_box[C][1][0], b20 = A.d_box[C][2][0];
             const int64_t d0 = A.d_box[C][0][1] - b00 + 1, d1n = A.d_box[C][1][1] - b10 + 1;
-            const int64_t q = (int64_t(k - b20) * d1n + (j - b10)) * d0 + (i - b00);
+            const int64_t q = (int64_t(P.k - b20) * d1n + (P.j - b10)) * d0 + (P.i - b00);
             inv = A.d_inv[C][q];
             eps = A.d_eps[C][q];
             const int np = A.d_np[C];
             for (int s = 0; s < np; ++s) {
-                const int64_t o = (int64_t(s) * 3 + C) * A.pstride + idx;
+                const int64_t o = (int64_t(s) * 3 + C) * A.pstride + P.o;
                 const double pc = double(A.Pcur[o]);
                 const double pp = double(A.Pprev[o]);
                 const double amp = A.d_amp[C][q * np + s];
@@ -192,16 +205,17 @@ __device__ __forceinline__ void update_point(const HalfArgs<T>& A, int i, int j,
         e_new = e_old + (A.dt * acc - dP) * inv;
     }
     const T stored = T(e_new);
-    u[idx] = stored;
+    A.u[C][P.o] = stored;
     // DFT monitors (k is uniform across the workgroup)
     for (int m = 0; m < A.nunits; ++m) {
         const DftDev& U = A.units[m];
-        if (U.comp != C || k < U.box[2][0] || k > U.box[2][1])
+        if (!U.fused || U.comp != C || P.k < U.box[2][0] || P.k > U.box[2][1])
             continue;
-        if (i < U.box[0][0] || i > U.box[0][1] || j < U.box[1][0] || j > U.box[1][1])
+        if (P.i < U.box[0][0] || P.i > U.box[0][1] || P.j < U.box[1][0] || P.j > U.box[1][1])
             continue;
         const double v = U.is_db ? eps * double(stored) + sumP : double(stored);
-        const int64_t off = (i - U.box[0][0]) + int64_t(U.dims[0]) * ((j - U.box[1][0]) + int64_t(U.dims[1]) * (k - U.box[2][0]));
+        const int64_t off = (P.i - U.box[0][0]) +
+                            int64_t(U.dims[0]) * ((P.j - U.box[1][0]) + int64_t(U.dims[1]) * (P.k - U.box[2][0]));
         const double* ph = A.ph + 2 * U.mf0;
         for (int f = 0; f < U.nf; ++f) {
             const int64_t o = int64_t(f) * U.npts + off;
@@ -211,81 +225,165 @@ __device__ __forceinline__ void update_point(const HalfArgs<T>& A, int i, int j,
     }
 }
 
-template <typename T, bool IS_E, bool MAT>
+// One half step over a (BX x BY) column tile and a z segment of A.lz planes.
+// Loads for U consecutive planes are issued before any of them is computed so
+// that each thread keeps several planes of requests in flight; the two
+// components differentiated along z are carried between planes in registers.
+template <typename T, bool IS_E, bool MAT, int U>
 __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
 {
-    const int i = A.dlo[0] + int(blockIdx.x) * BX + int(threadIdx.x);
-    const int j = A.dlo[1] + int(blockIdx.y) * BY + int(threadIdx.y);
-    const int kb = A.dlo[2] + int(blockIdx.z) * LZ;
-    const int ke = min(kb + LZ - 1, A.dhi[2]);
+    __shared__ MatEntry smat[MAT ? MAXMAT : 1];
+    if (MAT) {
+        for (int q = int(threadIdx.x + BX * threadIdx.y); q < A.nmats; q += BX * BY)
+            smat[q] = A.mats[q];
+        __syncthreads();
+    }
+    // XCD-aware order: consecutive dispatches land on the 8 XCDs in turn, so the
+    // tiles one XCD runs are made contiguous in space (shared halos stay in its L2)
+    const int b = int(blockIdx.x);
+    const int t = (b & 7) * (A.nblocks >> 3) + (b >> 3);
+    const int tx = t % A.gx;
+    const int tr = t / A.gx;
+    const int ty = tr % A.gy;
+    const int tz = tr / A.gy;
+    if (tz >= A.gz)
+        return;
+    const int i = A.dlo[0] + tx * BX + int(threadIdx.x);
+    const int j = A.dlo[1] + ty * BY + int(threadIdx.y);
+    const int kb = A.dlo[2] + tz * A.lz;
+    const int ke = min(kb + A.lz - 1, A.dhi[2]);
     if (i > A.dhi[0] || j > A.dhi[1])
         return;
-    const int64_t sy = A.pitch, sz = A.plane;
+    const int sy = A.pitch, sz = A.plane;
     const double inv_dx = A.inv_dx;
     const T* __restrict__ s0 = A.s[0];
     const T* __restrict__ s1 = A.s[1];
     const T* __restrict__ s2 = A.s[2];
+    const T* __restrict__ u0 = A.u[0];
+    const T* __restrict__ u1 = A.u[1];
+    const T* __restrict__ u2 = A.u[2];
     const bool in0 = i >= A.lo[0][0] && i <= A.hi[0][0] && j >= A.lo[0][1] && j <= A.hi[0][1];
     const bool in1 = i >= A.lo[1][0] && i <= A.hi[1][0] && j >= A.lo[1][1] && j <= A.hi[1][1];
     const bool in2 = i >= A.lo[2][0] && i <= A.hi[2][0] && j >= A.lo[2][1] && j <= A.hi[2][1];
-    int64_t idx = i + sy * j + sz * int64_t(kb);
-    // z carry: E needs s0,s1 at k and k+1; H needs them at k-1 and k
-    T cx, cy;
-    if (IS_E) {
-        cx = s0[idx];
-        cy = s1[idx];
-    } else {
-        cx = s0[idx - sz];
-        cy = s1[idx - sz];
+    PlaneCtx P;
+    P.i = i;
+    P.j = j;
+    P.sx = A.pml[0].W ? A.pml[0].smap[i] : -1;
+    P.sy = A.pml[1].W ? A.pml[1].smap[j] : -1;
+    P.bx = P.cx = P.by = P.cy = P.bz = P.cz = 0;
+    P.ikx = P.iky = P.ikz = 1;
+    if (P.sx >= 0) {
+        P.bx = A.pml[0].b[i];
+        P.cx = A.pml[0].c[i];
+        P.ikx = A.pml[0].ik[i];
     }
-    for (int k = kb; k <= ke; ++k, idx += sz) {
-        T nx, ny;
-        if (IS_E) {
-            nx = s0[idx + sz];
-            ny = s1[idx + sz];
-        } else {
-            nx = s0[idx];
-            ny = s1[idx];
-        }
-        const T c2 = s2[idx];
-        // component x: d/dy of s2, d/dz of s1
-        if (in0 && k >= A.lo[0][2] && k <= A.hi[0][2]) {
-            double d1, d2;
-            if (IS_E) {
-                d1 = (double(s2[idx + sy]) - double(c2)) * inv_dx;
-                d2 = (double(ny) - double(cy)) * inv_dx;
-            } else {
-                d1 = (double(s2[idx - sy]) - double(c2)) * inv_dx;
-                d2 = (double(cy) - double(ny)) * inv_dx;
+    if (P.sy >= 0) {
+        P.by = A.pml[1].b[j];
+        P.cy = A.pml[1].c[j];
+        P.iky = A.pml[1].ik[j];
+    }
+    const int base = i + sy * j;
+    T c0, c1;  // E: s0, s1 at plane k; H: at plane k-1
+    {
+        const int o = base + sz * kb;
+        c0 = IS_E ? s0[o] : s0[o - sz];
+        c1 = IS_E ? s1[o] : s1[o - sz];
+    }
+    for (int kc = kb; kc <= ke; kc += U) {
+        T n0[U], n1[U], v2[U], v2a[U], v2b[U], v1a[U], v0b[U], e0[U], e1[U], e2[U];
+        uint32_t idv[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int o = base + sz * (kc + q);
+            if (kc + q <= ke) {
+                if (IS_E) {
+                    n0[q] = s0[o + sz];
+                    n1[q] = s1[o + sz];
+                    v2a[q] = s2[o + 1];
+                    v2b[q] = s2[o + sy];
+                    v1a[q] = s1[o + 1];
+                    v0b[q] = s0[o + sy];
+                } else {
+                    n0[q] = s0[o];
+                    n1[q] = s1[o];
+                    v2a[q] = s2[o - 1];
+                    v2b[q] = s2[o - sy];
+                    v1a[q] = s1[o - 1];
+                    v0b[q] = s0[o - sy];
+                }
+                v2[q] = s2[o];
+                e0[q] = u0[o];
+                e1[q] = u1[o];
+                e2[q] = u2[o];
+                idv[q] = MAT ? A.ids[o] : 0u;
             }
-            update_point<T, MAT, 0>(A, i, j, k, idx, d1, d2);
         }
-        // component y: d/dz of s0, d/dx of s2
-        if (in1 && k >= A.lo[1][2] && k <= A.hi[1][2]) {
-            double d1, d2;
-            if (IS_E) {
-                d1 = (double(nx) - double(cx)) * inv_dx;
-                d2 = (double(s2[idx + 1]) - double(c2)) * inv_dx;
-            } else {
-                d1 = (double(cx) - double(nx)) * inv_dx;
-                d2 = (double(s2[idx - 1]) - double(c2)) * inv_dx;
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int k = kc + q;
+            if (k > ke)
+                break;
+            P.k = k;
+            P.o = base + sz * k;
+            P.sz = A.pml[2].W ? A.pml[2].smap[k] : -1;
+            if (P.sz >= 0) {
+                P.bz = A.pml[2].b[k];
+                P.cz = A.pml[2].c[k];
+                P.ikz = A.pml[2].ik[k];
             }
-            update_point<T, MAT, 1>(A, i, j, k, idx, d1, d2);
-        }
-        // component z: d/dx of s1, d/dy of s0
-        if (in2 && k >= A.lo[2][2] && k <= A.hi[2][2]) {
-            double d1, d2;
-            if (IS_E) {
-                d1 = (double(s1[idx + 1]) - double(cy)) * inv_dx;
-                d2 = (double(s0[idx + sy]) - double(cx)) * inv_dx;
-            } else {
-                d1 = (double(s1[idx - 1]) - double(ny)) * inv_dx;
-                d2 = (double(s0[idx - sy]) - double(nx)) * inv_dx;
+            // E: s0(k)=c0, s0(k+1)=n0; H: s0(k-1)=c0, s0(k)=n0 (s1 likewise)
+            const double a0 = double(c0), a1 = double(c1), g0 = double(n0[q]), g1 = double(n1[q]);
+            const double w2 = double(v2[q]);
+            if (in0 && k >= A.lo[0][2] && k <= A.hi[0][2]) {
+                const double d1 = (double(v2b[q]) - w2) * inv_dx;
+                const double d2 = (IS_E ? (g1 - a1) : (a1 - g1)) * inv_dx;
+                update_point<T, MAT, 0>(A, smat, P, d1, d2, e0[q], idv[q]);
             }
-            update_point<T, MAT, 2>(A, i, j, k, idx, d1, d2);
+            if (in1 && k >= A.lo[1][2] && k <= A.hi[1][2]) {
+                const double d1 = (IS_E ? (g0 - a0) : (a0 - g0)) * inv_dx;
+                const double d2 = (double(v2a[q]) - w2) * inv_dx;
+                update_point<T, MAT, 1>(A, smat, P, d1, d2, e1[q], idv[q]);
+            }
+            if (in2 && k >= A.lo[2][2] && k <= A.hi[2][2]) {
+                const double d1 = (double(v1a[q]) - (IS_E ? a1 : g1)) * inv_dx;
+                const double d2 = (double(v0b[q]) - (IS_E ? a0 : g0)) * inv_dx;
+                update_point<T, MAT, 2>(A, smat, P, d1, d2, e2[q], idv[q]);
+            }
+            c0 = n0[q];
+            c1 = n1[q];
         }
-        cx = nx;
-        cy = ny;
+    }
+}
+
+// DFT units that are too narrow along x to share the half-step kernel's
+// coalesced rows (planes normal to x): one thread per point, run after the
+// point sources so they read final values.
+template <typename T>
+struct SparseDftArgs {
+    const T* u[3];
+    int pitch, plane;
+    int nunits;
+    DftDev units[MAXU];
+    const double* ph;
+};
+
+template <typename T>
+__global__ void k_dft_sparse(const SparseDftArgs<T> A)
+{
+    const DftDev& U = A.units[blockIdx.y];
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= U.npts)
+        return;
+    const int ii = int(t % U.dims[0]);
+    const int jj = int((t / U.dims[0]) % U.dims[1]);
+    const int kk = int(t / (int64_t(U.dims[0]) * U.dims[1]));
+    const int o = (U.box[0][0] + ii) + A.pitch * (U.box[1][0] + jj) + A.plane * (U.box[2][0] + kk);
+    const double v = double(A.u[U.comp][o]);
+    const double* ph = A.ph + 2 * U.mf0;
+    for (int f = 0; f < U.nf; ++f) {
+        const int64_t q = int64_t(f) * U.npts + t;
+        U.im[q] += ph[2 * f + 1] * v;
+        U.re[q] += ph[2 * f] * v;
     }
 }
 
@@ -376,6 +474,42 @@ __global__ void k_faces(const FaceArgs<T> A)
     u[d] = T(F.mult * double(u[s]));
 }
 
+// ---------------------------------------------------------------- stop-condition samples
+// sum over a DFT unit's points of re^2 + im^2 at one frequency
+// (DFT_Unit::get_local_norm, src/dft.cpp:93-107); one workgroup, result to out[slot]
+__global__ void __launch_bounds__(1024) k_dft_norm(const double* re, const double* im, int64_t n, double* out,
+                                                   int64_t slot)
+{
+    __shared__ double part[16];
+    double s = 0;
+    for (int64_t q = threadIdx.x; q < n; q += blockDim.x)
+        s += re[q] * re[q] + im[q] * im[q];
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off);
+    const int w = int(threadIdx.x) / 64;
+    if ((threadIdx.x & 63) == 0)
+        part[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int q = 0; q < int(blockDim.x) / 64; ++q)
+            t += part[q];
+        out[slot] = t;
+    }
+}
+
+// interpolated field value at a point (Fields::get_eh_helper, src/fields.cpp:245-248)
+template <typename T>
+__global__ void k_point_value(const T* u, const int64_t* idx, const double* w, int nterm, double* out, int64_t slot)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double v = 0;
+        for (int q = 0; q < nterm; ++q)
+            v += double(u[idx[q]]) * w[q];
+        out[slot] = v;
+    }
+}
+
 // ---------------------------------------------------------------- probes / phasors
 template <typename T>
 struct ProbeArgs {
@@ -433,9 +567,11 @@ struct EngineImpl {
     void* Pa[2] = {nullptr, nullptr};
     void* Pb[2] = {nullptr, nullptr};
     int S_slots[2] = {0, 0};
-    uint8_t* ids[2][3] = {};
-    double *m_inv[2] = {}, *m_eps[2] = {}, *p_c1[2] = {}, *p_c2[2] = {}, *p_sc3[2] = {};
-    int *m_np[2] = {}, *m_p0[2] = {};
+    uint32_t* ids[2] = {};  // packed ids of the three components of a half
+    MatEntry* mats[2] = {};
+    int nmats[2] = {0, 0};
+    double *p_c1[2] = {}, *p_c2[2] = {}, *p_sc3[2] = {};
+    int lz = 32;
     double *d_inv[2][3] = {}, *d_eps[2][3] = {}, *d_amp[2][3] = {}, *d_c[2][3] = {};
     PmlDev pml[2][3] = {};
     std::vector<DftDev> units[2];
@@ -477,6 +613,23 @@ struct EngineImpl {
             HIP_OK(hipMemsetAsync(p, 0, count * sizeof(X), stream));
         return static_cast<X*>(p);
     }
+    // reusable device buffer for per-step stop-condition samples
+    double* samples = nullptr;
+    size_t samples_cap = 0;
+    double* sample_buf(size_t n)
+    {
+        n = std::max<size_t>(n, 1);
+        if (n > samples_cap) {
+            HIP_OK(hipStreamSynchronize(stream));
+            if (samples) {
+                allocs.erase(std::find(allocs.begin(), allocs.end(), static_cast<void*>(samples)));
+                HIP_OK(hipFree(samples));
+            }
+            samples = dalloc<double>(n);
+            samples_cap = n;
+        }
+        return samples;
+    }
     template <typename X>
     X* upload(const std::vector<X>& v)
     {
@@ -496,6 +649,16 @@ struct EngineImpl {
     HalfArgs<T> half_args(int h);
     template <typename T>
     void launch_half(int h, hipEvent_t e0, hipEvent_t e1);
+    template <typename T, bool IS_E, bool MAT>
+    void launch_k(int U, dim3 grid, dim3 block, const HalfArgs<T>& A)
+    {
+        switch (U) {
+        case 1: k_half<T, IS_E, MAT, 1><<<grid, block, 0, stream>>>(A); break;
+        case 2: k_half<T, IS_E, MAT, 2><<<grid, block, 0, stream>>>(A); break;
+        default: k_half<T, IS_E, MAT, 4><<<grid, block, 0, stream>>>(A); break;
+        }
+    }
+    int unroll = 4, unroll_mat = 4;  // z planes whose loads are issued together
     void ensure_phasors();
     template <typename T>
     void step_once(hipEvent_t* ev, bool timed_e, bool timed_h);
@@ -519,6 +682,21 @@ void EngineImpl::build()
     const Layout& L = s.L;
     fp32 = s.precision == "fp32";
     tsz = fp32 ? 4 : 8;
+    // measured on MI355X (256^3 dispersive workload): short z segments give more
+    // workgroups in flight; the material kernel prefers one plane per batch
+    // (fewer live registers, higher occupancy)
+    lz = 8;
+    unroll = fp32 ? 4 : 2;
+    unroll_mat = 1;
+    // tuning overrides: z segment per workgroup, planes per load batch
+    if (const char* e = std::getenv("FFIP_LZ"))
+        lz = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("FFIP_UNROLL"))
+        unroll = std::atoi(e);
+    if (const char* e = std::getenv("FFIP_UNROLL_MAT"))
+        unroll_mat = std::atoi(e);
+    if (L.alloc >= (int64_t(1) << 31))
+        throw std::runtime_error("grid too large for 32-bit cell offsets");
     HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (int c = 0; c < 6; ++c) {
         field[c] = fp32 ? (void*)dalloc<float>(L.alloc) : (void*)dalloc<double>(L.alloc);
@@ -534,12 +712,16 @@ void EngineImpl::build()
             Pb[h] = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
         }
         if (!M.uniform) {
+            std::vector<uint32_t> packed(size_t(L.alloc), 0u);
             for (int c = 0; c < 3; ++c)
-                ids[h][c] = upload(M.ids[c]);
-            m_inv[h] = upload(M.table.inv_eps);
-            m_eps[h] = upload(M.table.eps);
-            m_np[h] = upload(M.table.npoles);
-            m_p0[h] = upload(M.table.pole0);
+                for (size_t q = 0; q < packed.size(); ++q)
+                    packed[q] |= uint32_t(M.ids[c][q]) << (8 * c);
+            ids[h] = upload(packed);
+            std::vector<MatEntry> me(M.table.eps.size());
+            for (size_t e = 0; e < me.size(); ++e)
+                me[e] = {M.table.inv_eps[e], M.table.eps[e], M.table.npoles[e], M.table.pole0[e]};
+            nmats[h] = int(me.size());
+            mats[h] = upload(me);
             p_c1[h] = upload(M.table.c1);
             p_c2[h] = upload(M.table.c2);
             p_sc3[h] = upload(M.table.sc3);
@@ -603,6 +785,10 @@ void EngineImpl::build()
                 }
                 d.npts = 0;
             }
+            // rows shorter than 16 points (e.g. planes normal to x) go to the
+            // separate one-thread-per-point kernel; D/B samples need the
+            // polarization sums, so those always stay in the half-step kernel
+            d.fused = (d.is_db || d.dims[0] >= 16) ? 1 : 0;
             d.re = dalloc<double>(size_t(d.npts) * d.nf);
             d.im = dalloc<double>(size_t(d.npts) * d.nf);
             units[h].push_back(d);
@@ -669,7 +855,7 @@ void EngineImpl::build()
                 eoff.push_back(int(amp.size()));
                 for (size_t u = 0; u < s.units[h].size(); ++u) {
                     const DftUnit& U = s.units[h][u];
-                    if (!U.has_local || U.comp % 3 != key[t].second)
+                    if (!U.has_local || U.comp % 3 != key[t].second || !units[h][u].fused)
                         continue;
                     if (D0.i < U.box[0][0] || D0.i > U.box[0][1] || D0.j < U.box[1][0] || D0.j > U.box[1][1] ||
                         D0.k < U.box[2][0] || D0.k > U.box[2][1])
@@ -746,10 +932,11 @@ void EngineImpl::build()
             }
         }
         for (auto& U : units[h])
-            dftb += double(U.npts) * U.nf * 32.0;  // re, im: read + write, FP64
+            if (U.fused)  // the others are accumulated by k_dft_sparse
+                dftb += double(U.npts) * U.nf * 32.0;  // re, im: read + write, FP64
         double* t = traffic[h];
         t[0] = (2 * nreal + nsrc) * double(tsz);
-        t[1] = M.uniform ? 0.0 : nreal;
+        t[1] = M.uniform ? 0.0 : 4.0 * nreal / 3.0;  // one packed 32-bit id word per cell
         t[2] = ade * double(tsz);
         t[3] = psi * double(tsz);
         t[4] = dftb;
@@ -789,8 +976,8 @@ HalfArgs<T> EngineImpl::half_args(int h)
         A.u[c] = static_cast<T*>(field[h * 3 + c]) + L.origin;
         A.s[c] = static_cast<const T*>(field[(1 - h) * 3 + c]) + L.origin;
     }
-    A.pitch = L.pitch;
-    A.plane = L.plane;
+    A.pitch = int(L.pitch);
+    A.plane = int(L.plane);
     for (int a = 0; a < 3; ++a) {
         A.n[a] = L.n[a];
         A.dlo[a] = 1 << 30;
@@ -811,18 +998,14 @@ HalfArgs<T> EngineImpl::half_args(int h)
     A.u_inv = M.u_inv;
     A.u_eps = M.u_eps;
     if (!M.uniform) {
-        for (int c = 0; c < 3; ++c)
-            A.ids[c] = ids[h][c] + L.origin;
-        A.m_inv = m_inv[h];
-        A.m_eps = m_eps[h];
-        A.m_np = m_np[h];
-        A.m_p0 = m_p0[h];
+        A.ids = ids[h] + L.origin;
+        A.mats = mats[h];
+        A.nmats = nmats[h];
         A.p_c1 = p_c1[h];
         A.p_c2 = p_c2[h];
         A.p_sc3 = p_sc3[h];
         for (int c = 0; c < 3; ++c) {
             const DenseRegion& D = M.dense[c];
-            A.d_present[c] = D.present ? 1 : 0;
             if (!D.present)
                 continue;
             for (int a = 0; a < 3; ++a) {
@@ -847,6 +1030,14 @@ HalfArgs<T> EngineImpl::half_args(int h)
     for (int u = 0; u < A.nunits; ++u)
         A.units[u] = units[h][u];
     A.ph = ph[h] + size_t(n - ph_start) * 2 * std::max(1, MF[h]);
+    // tile grid, padded to a multiple of the 8 XCDs
+    A.lz = lz;
+    if (A.dhi[0] >= A.dlo[0] && A.dhi[1] >= A.dlo[1] && A.dhi[2] >= A.dlo[2]) {
+        A.gx = (A.dhi[0] - A.dlo[0] + BX) / BX;
+        A.gy = (A.dhi[1] - A.dlo[1] + BY) / BY;
+        A.gz = (A.dhi[2] - A.dlo[2] + lz) / lz;
+        A.nblocks = ((A.gx * A.gy * A.gz + 7) / 8) * 8;
+    }
     return A;
 }
 
@@ -873,20 +1064,21 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
     if (A.dhi[0] < A.dlo[0] || A.dhi[1] < A.dlo[1] || A.dhi[2] < A.dlo[2])
         return;
     dim3 block(BX, BY, 1);
-    dim3 grid((A.dhi[0] - A.dlo[0] + BX) / BX, (A.dhi[1] - A.dlo[1] + BY) / BY, (A.dhi[2] - A.dlo[2] + LZ) / LZ);
+    dim3 grid(A.nblocks, 1, 1);
     const bool mat = !S->mat[h].uniform;
+    const int U = mat ? unroll_mat : unroll;
     if (e0)
         HIP_OK(hipEventRecord(e0, stream));
     if (h == 0) {
         if (mat)
-            k_half<T, true, true><<<grid, block, 0, stream>>>(A);
+            launch_k<T, true, true>(U, grid, block, A);
         else
-            k_half<T, true, false><<<grid, block, 0, stream>>>(A);
+            launch_k<T, true, false>(U, grid, block, A);
     } else {
         if (mat)
-            k_half<T, false, true><<<grid, block, 0, stream>>>(A);
+            launch_k<T, false, true>(U, grid, block, A);
         else
-            k_half<T, false, false><<<grid, block, 0, stream>>>(A);
+            launch_k<T, false, false>(U, grid, block, A);
     }
     HIP_OK(hipGetLastError());
     if (e1)
@@ -919,6 +1111,25 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
         B.ph = A.ph;
         k_sources<T><<<(SR.ntarget + 255) / 256, 256, 0, stream>>>(B, t);
         HIP_OK(hipGetLastError());
+    }
+    {
+        SparseDftArgs<T> D;
+        std::memset(&D, 0, sizeof(D));
+        for (int c = 0; c < 3; ++c)
+            D.u[c] = static_cast<const T*>(field[h * 3 + c]) + S->L.origin;
+        D.pitch = int(S->L.pitch);
+        D.plane = int(S->L.plane);
+        int64_t most = 0;
+        for (auto& U : units[h])
+            if (!U.fused && U.npts > 0) {
+                D.units[D.nunits++] = U;
+                most = std::max(most, U.npts);
+            }
+        D.ph = A.ph;
+        if (D.nunits > 0) {
+            k_dft_sparse<T><<<dim3(unsigned((most + 255) / 256), D.nunits), 256, 0, stream>>>(D);
+            HIP_OK(hipGetLastError());
+        }
     }
     if (!faces[h].empty()) {
         FaceArgs<T> F;
@@ -1204,14 +1415,167 @@ double Engine::probe(const double pos[3], int ct)
     return v;
 }
 
-int64_t Engine::run_until_dft(const double*, const double*, int, double, double, double, std::string*)
+// Steps in examination windows with one per-step sample kept on the device and
+// read back once per window; `sample` enqueues the sample of the step just taken
+// into buf[slot].
+template <typename F>
+static std::vector<double> window_samples(EngineImpl& p, int64_t nsteps, double* buf, F&& sample)
 {
-    throw std::runtime_error("run_until_dft is not available yet");
+    for (int64_t q = 0; q < nsteps; ++q) {
+        p.step_n(1);
+        sample(q);
+    }
+    std::vector<double> h(size_t(std::max<int64_t>(nsteps, 0)));
+    if (nsteps > 0) {
+        HIP_OK(hipStreamSynchronize(p.stream));
+        HIP_OK(hipMemcpy(h.data(), buf, size_t(nsteps) * 8, hipMemcpyDeviceToHost));
+    }
+    return h;
 }
 
-int64_t Engine::run_until_decay(const double*, int, double, double, std::string*)
+static int64_t steps_until(const EngineImpl& p, double t)
 {
-    throw std::runtime_error("run_until_fields_decayed is not available yet");
+    // number of steps the loop `while (time_step * dt < t) step` takes from now
+    int64_t n = p.n, k = 0;
+    while (double(n + k) * p.S->dt < t)
+        ++k;
+    return k;
+}
+
+// Simulation::run_until_dft, src/simulation.cpp:782-842
+int64_t Engine::run_until_dft(const double c[3], const double s[3], int ct, double freq, double interval, double var,
+                              std::string* log)
+{
+    EngineImpl& P = *p_;
+    const Setup& S = *P.S;
+    double p1[3], p2[3];
+    for (int a = 0; a < 3; ++a) {
+        p1[a] = (c[a] - s[a] / 2) / S.dx2;
+        p2[a] = (c[a] + s[a] / 2) / S.dx2;
+    }
+    // closure of the requested box, then the registered unit with that closure
+    auto lo = [](double x, int T) {
+        int xi = int(std::floor(x));
+        return ((xi & 1) ^ T) ? xi - 1 : xi;
+    };
+    auto hi = [](double x, int T) {
+        int xi = int(std::ceil(x));
+        return ((xi & 1) ^ T) ? xi + 1 : xi;
+    };
+    int q1[3], q2[3];
+    for (int a = 0; a < 3; ++a) {
+        int T = (ct >> a) & 1;
+        q1[a] = lo(p1[a], T);
+        q2[a] = hi(p2[a], T);
+    }
+    const int h = ctype_is_e(ct) ? 0 : 1;
+    int u = -1;
+    for (size_t k = 0; k < S.units[h].size(); ++k) {
+        const DftUnit& U = S.units[h][k];
+        if (U.ct == ct && U.q1[0] == q1[0] && U.q1[1] == q1[1] && U.q1[2] == q1[2] && U.q2[0] == q2[0] &&
+            U.q2[1] == q2[1] && U.q2[2] == q2[2])
+            u = int(k);
+    }
+    if (u < 0)
+        throw std::runtime_error("dft is not registered");
+    const DftUnit& U = S.units[h][u];
+    auto it = std::lower_bound(U.freqs.begin(), U.freqs.end(), freq);
+    if (it == U.freqs.end() || *it != freq)
+        throw std::runtime_error("frequency is not registered");
+    const int64_t fi = it - U.freqs.begin();
+    const DftDev& D = P.units[h][u];
+    const int64_t start = P.n;
+    const double turnoff = S.turnoff_time;
+    P.step_n(steps_until(P, turnoff));
+    int extension = 1;
+    double cur_var;
+    do {
+        double max_norm = std::numeric_limits<double>::min();
+        double min_norm = std::numeric_limits<double>::max();
+        const int64_t k = steps_until(P, turnoff + extension * interval);
+        double* buf = P.sample_buf(size_t(k));
+        auto vals = window_samples(P, k, buf, [&](int64_t q) {
+            k_dft_norm<<<1, 1024, 0, P.stream>>>(D.re + fi * D.npts, D.im + fi * D.npts, D.npts, buf, q);
+            HIP_OK(hipGetLastError());
+        });
+        for (double v : vals) {
+            if (v > max_norm)
+                max_norm = v;
+            if (v < min_norm)
+                min_norm = v;
+        }
+        ++extension;
+        cur_var = (max_norm - min_norm) / max_norm;
+        if (log) {
+            char b[160];
+            std::snprintf(b, sizeof b, "max norm=%.10g,min norm=%.10g,var=%.10g\n", max_norm, min_norm, cur_var);
+            *log += b;
+        }
+    } while (cur_var > var);
+    sync();
+    return P.n - start;
+}
+
+// Simulation::run_until_fields_decayed, src/simulation.cpp:844-901
+int64_t Engine::run_until_decay(const double pos[3], int ct, double interval, double decay_by, std::string* log)
+{
+    EngineImpl& P = *p_;
+    const Setup& S = *P.S;
+    if (!ctype_is_eh(ct))
+        throw std::runtime_error("decay probe takes an E or H component");
+    auto terms = probe_terms(S, pos, ct);
+    std::vector<int64_t> ti;
+    std::vector<double> tw;
+    int comp = comp_index(ct);
+    for (auto& t : terms) {
+        ti.push_back(t.index);
+        tw.push_back(t.w);
+    }
+    int64_t* d_idx = P.upload(ti);
+    double* d_w = P.upload(tw);
+    const int nterm = int(ti.size());
+    auto sample = [&](double* buf) {
+        return [&P, buf, d_idx, d_w, nterm, comp](int64_t q) {
+            if (P.fp32)
+                k_point_value<float><<<1, 64, 0, P.stream>>>(static_cast<const float*>(P.field[comp]), d_idx, d_w,
+                                                             nterm, buf, q);
+            else
+                k_point_value<double><<<1, 64, 0, P.stream>>>(static_cast<const double*>(P.field[comp]), d_idx,
+                                                              d_w, nterm, buf, q);
+            HIP_OK(hipGetLastError());
+        };
+    };
+    const int64_t start = P.n;
+    const double turnoff = S.turnoff_time;
+    double max_sqr = 0;
+    {
+        const int64_t k = steps_until(P, turnoff);
+        double* buf = P.sample_buf(size_t(k));
+        for (double v : window_samples(P, k, buf, sample(buf)))
+            if (v * v > max_sqr)
+                max_sqr = v * v;
+    }
+    int extension = 1;
+    double decay;
+    do {
+        double cur_max = 0;
+        const int64_t k = steps_until(P, turnoff + extension * interval);
+        double* buf = P.sample_buf(size_t(k));
+        for (double v : window_samples(P, k, buf, sample(buf)))
+            if (v * v > cur_max)
+                cur_max = v * v;
+        ++extension;
+        if (cur_max > max_sqr)
+            max_sqr = cur_max;
+        decay = cur_max / max_sqr;
+        if (log) {
+            char b[160];
+            std::snprintf(b, sizeof b, "Field Decay = %.10g/%.10g=%.10g\n", cur_max, max_sqr, decay);
+            *log += b;
+        }
+    } while (decay > decay_by);
+    sync();
+    return P.n - start;
 }
 
 }  // namespace ffip_amd

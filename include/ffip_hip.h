@@ -113,6 +113,13 @@ int ffip_sim_info(const ffip_sim *sim, double out[6]);
 
 void ffip_sim_destroy(ffip_sim *sim);
 
+/* Adjoint of zero-padded trilinear interpolation (replaces the pybind helper
+ * _ffip.transpose, app/ffip.cpp:308-352): scatters pts[i] = (x, y, z, value) onto
+ * the regular grid gx[nx] x gy[ny] x gz[nz] (uniform spacing taken from the first
+ * two nodes) and writes out[nz][ny][nx] (overwritten, zero-initialised). */
+int ffip_transpose(const double *gx, int64_t nx, const double *gy, int64_t ny, const double *gz, int64_t nz,
+                   const double *pts, int64_t npts, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -974,9 +974,84 @@ class OracleSim:
         return int(_lib().osim_run_until_time(ctypes.byref(self._sim), float(t)))
 
     def run(self, stop):
-        if stop["type"] != "time":
-            raise ValueError("the oracle drives only the 'time' stop condition")
-        return self.run_until_time(float(stop["time"]))
+        """Simulation::run dispatch, src/simulation.cpp:903-949."""
+        t0 = self.time_step
+        if stop["type"] == "time":
+            self.run_until_time(float(stop["time"]))
+        elif stop["type"] == "decay":
+            self.run_until_fields_decayed(float(stop["time interval examined"]), stop["position"],
+                                          stop["field component"], float(stop["decayed by"]))
+        else:
+            self.run_until_dft(stop["center"], stop["size"], stop["field component"], float(stop["frequency"]),
+                               float(stop["time interval examined"]), float(stop["variation"]))
+        return self.time_step - t0
+
+    def run_until_dft(self, center, size, comp, freq, interval, var):
+        """src/simulation.cpp:782-842 (single rank: the all-reduce is the identity)."""
+        dx2 = self.dx2
+        p1 = [(float(center[a]) - float(size[a]) / 2) / dx2 for a in range(3)]
+        p2 = [(float(center[a]) + float(size[a]) / 2) / dx2 for a in range(3)]
+        ct = CTYPE[comp]
+        q1, q2 = component_closure(p1, p2, ct)
+        half = 0 if is_e_point(ct) else 1
+        unit = None
+        for u in self.units[half]:
+            if u["ct"] == ct and u["p1"] == q1 and u["p2"] == q2:
+                unit = u
+        if unit is None:
+            raise ValueError("dft is not registered")
+        if freq not in unit["freqs"]:
+            raise ValueError("frequency is not registered")
+        fi = unit["freqs"].index(freq)
+        n = unit["local_coords"].shape[0]
+
+        def norm():  # DFT_Unit::get_local_norm, src/dft.cpp:93-107
+            re = unit["re"][fi * n:(fi + 1) * n]
+            im = unit["im"][fi * n:(fi + 1) * n]
+            return float(np.sum(re * re + im * im))
+
+        turnoff = self.turnoff_time
+        while self.time_step * self.dt < turnoff:
+            self.run_steps(1)
+        extension = 1
+        self.stop_log = []
+        while True:
+            mx, mn = np.finfo(float).tiny, np.finfo(float).max
+            while self.time_step * self.dt < turnoff + extension * interval:
+                self.run_steps(1)
+                v = norm()
+                mx, mn = max(mx, v), min(mn, v)
+            extension += 1
+            cur = (mx - mn) / mx
+            self.stop_log.append((mx, mn, cur))
+            if not cur > var:
+                break
+
+    def run_until_fields_decayed(self, interval, pos, comp, decay_by):
+        """src/simulation.cpp:844-901."""
+        terms = self.grid.interp_terms([float(v) / self.dx2 for v in pos], CTYPE[comp])
+
+        def value():
+            return sum(self.eh[i] * w for i, w in terms)
+
+        turnoff = self.turnoff_time
+        max_sqr = 0.0
+        while self.time_step * self.dt < turnoff:
+            self.run_steps(1)
+            max_sqr = max(max_sqr, value() ** 2)
+        extension = 1
+        self.stop_log = []
+        while True:
+            cur = 0.0
+            while self.time_step * self.dt < turnoff + extension * interval:
+                self.run_steps(1)
+                cur = max(cur, value() ** 2)
+            extension += 1
+            max_sqr = max(max_sqr, cur)
+            decay = cur / max_sqr
+            self.stop_log.append((cur, max_sqr, decay))
+            if not decay > decay_by:
+                break
 
     def dft_output(self, k):
         """Volume_Fields_DFT::output -> output_fields_collective (src/dft.cpp:273-320):

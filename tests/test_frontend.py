@@ -1,0 +1,113 @@
+"""Front-end mirror (ffip_amd.Simulation and friends), the native transpose and
+the data-dependent stop conditions.
+
+CPU tests: JSON produced by the mirror, transpose against the adjoint identity
+<I x, y> = <x, I^T y> with scipy's trilinear interpolation as I, and the
+reference's error behaviour.  GPU tests: Simulation.run against the oracle and
+the run_until_dft / run_until_fields_decayed stop conditions (same step count,
+same DFT results)."""
+import numpy as np
+import pytest
+from scipy.interpolate import RegularGridInterpolator
+
+import ffip_amd as ffip
+from ffip_amd import workloads as W
+from oracle.oracle import OracleSim
+
+from .conftest import rel_l2
+
+
+def _dipole_sim(n=20, precision="fp64"):
+    sim = ffip.Simulation(size=ffip.Vector3(n, n, n), resolution=1, pmls=[ffip.PML(4)],
+                          sources=[ffip.Source(ffip.Gaussian1(0.1), center=ffip.Vector3(),
+                                               size=ffip.Vector3(), field_component="Ez")],
+                          precision=precision)
+    dft = sim.add_dft_fields(center=ffip.Vector3(z=3), size=ffip.Vector3(n, n, 0), frequency=[0.12, 0.08],
+                             field_component="Ez")
+    return sim, dft
+
+
+def test_json_matches_reference_schema():
+    sim, _ = _dipole_sim()
+    j = sim.get_json(ffip.run_until_time(50))
+    # Simulation sets sigma max = 0.8 (order + 1) resolution (ffip/simulation.py:322-324)
+    assert j["PML"] == [{"thickness": 4.0, "k max": 1.0, "order": 3, "sigma max": 3.2}]
+    assert j["sources"][0]["function"] == {"type": "Gaussian1", "frequency": 0.1, "cutoff": 4.0, "start time": 0.0}
+    assert j["fields output"][0]["frequency"] == [0.08, 0.12]  # sorted, as Fields_DFT stores them
+    assert j["fields output"][0]["output group"] == "volume fields dft 0"
+    assert j["stop condition"] == {"type": "time", "time": 50.0}
+    m = ffip.Medium(epsilon=2, E_conductivity=0.1, E_susceptibilities=[ffip.LorentzianSusceptibility(0.2, 0.01,
+                                                                                                     sigma=0.5)])
+    assert m.get_json()["electric susceptibility"] == [{"type": "Lorentz", "frequency": 0.2, "gamma": 0.01,
+                                                       "amplitude": 0.5}]
+
+
+def test_transpose_is_adjoint_of_trilinear_interpolation():
+    rng = np.random.default_rng(1)
+    gx, gy, gz = np.linspace(-2, 3, 6), np.linspace(0, 2, 5), np.linspace(-1, 1, 4)
+    pts = np.stack([rng.uniform(-2, 3, 40), rng.uniform(0, 2, 40), rng.uniform(-1, 1, 40)], axis=1)
+    vals = rng.normal(size=40)
+    field = rng.normal(size=(gz.size, gy.size, gx.size))
+    interp = RegularGridInterpolator((gz, gy, gx), field)(np.stack([pts[:, 2], pts[:, 1], pts[:, 0]], axis=1))
+    tr = ffip.transpose(gx, gy, gz, np.concatenate([pts, vals[:, None]], axis=1))
+    assert abs(np.dot(interp, vals) - np.sum(tr * field)) < 1e-12 * np.abs(tr * field).sum()
+    # zero padding: a point half a spacing beyond the first node keeps half its value
+    out = ffip.transpose(gx, gy, gz, np.array([[gx[0] - 0.5, gy[0], gz[0], 2.0]]))
+    assert out[0, 0, 0] == pytest.approx(1.0) and out.sum() == pytest.approx(1.0)
+
+
+def test_transpose_errors_follow_reference():
+    with pytest.raises(KeyError):
+        ffip.transpose(np.zeros((2, 2)), [0, 1], [0], np.zeros((1, 4)))
+    with pytest.raises(KeyError):
+        ffip.transpose([], [0, 1], [0], np.zeros((1, 4)))
+    with pytest.raises(ValueError):
+        ffip.transpose([0, 1], [0, 1], [0], np.zeros((1, 3)))
+
+
+def test_oracle_stop_conditions_terminate():
+    cfg = W.vacuum_dipole(n=16, pml_cells=4, plane_z=3)
+    o = OracleSim(cfg)
+    n = o.run({"type": "dft", "center": [0, 0, 3], "size": [16, 16, 0], "field component": "Ez",
+               "time interval examined": 10.0, "variation": 1e-3, "frequency": 0.1})
+    assert n > 26 and o.stop_log[-1][2] <= 1e-3
+    o = OracleSim(cfg)
+    n = o.run({"type": "decay", "position": [0, 0, 3], "field component": "Ez", "time interval examined": 10.0,
+               "decayed by": 1e-4})
+    assert n > 26 and o.stop_log[-1][2] <= 1e-4
+
+
+@pytest.mark.gpu
+def test_simulation_run_matches_oracle():
+    sim, dft = _dipole_sim()
+    stop = ffip.run_until_time(60)
+    sim.run(stop, np=12)
+    cfg = sim.get_json(stop)
+    o = OracleSim(cfg)
+    o.run(cfg["stop condition"])
+    ref = o.dft_output(0)
+    assert dft.values.shape == ref["real"].shape
+    assert rel_l2(dft.values, ref["real"] + 1j * ref["imag"]) <= 1e-10
+    assert np.array_equal(dft.frequency, ref["f"])
+    # interpolation on the read fields (Fields_DFT.__call__)
+    v = dft(0.12, [3.0], [0.0, 1.0], [0.0])
+    assert v.shape == (1, 1, 2, 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stop", [
+    {"type": "dft", "center": [0, 0, 3], "size": [16, 16, 0], "field component": "Ez",
+     "time interval examined": 10.0, "variation": 1e-3, "frequency": 0.1},
+    {"type": "decay", "position": [0, 0, 3], "field component": "Ez", "time interval examined": 10.0,
+     "decayed by": 1e-4},
+])
+def test_stop_conditions_match_oracle(stop):
+    cfg = W.vacuum_dipole(n=16, pml_cells=4, plane_z=3)
+    o = OracleSim(cfg)
+    n_o = o.run(stop)
+    from ffip_amd._lib import LibSim
+    g = LibSim(cfg)
+    n_g = g.run(stop)
+    assert n_g == n_o
+    a, b = g.dft(0), o.dft_output(0)
+    assert rel_l2(a["real"], b["real"]) <= 1e-10 and rel_l2(a["imag"], b["imag"]) <= 1e-10
