@@ -52,6 +52,8 @@ _SIGS = {
     "ffip_sim_traffic": ([ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     "ffip_sim_plan": ([ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)],
                       ctypes.c_int),
+    "ffip_adjoint_gradient": ([ctypes.c_void_p, ctypes.c_void_p] + [ctypes.c_void_p] * 2 + [ctypes.c_double] +
+                              [ctypes.c_void_p] * 6, ctypes.c_int),
     "ffip_transpose": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                         ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
     "ffip_sim_destroy": ([ctypes.c_void_p], None),
@@ -100,6 +102,21 @@ def transpose(gx, gy, gz, pt):
     out = np.zeros((gz.size, gy.size, gx.size))
     _check(lib().ffip_transpose(_ptr(gx), gx.size, _ptr(gy), gy.size, _ptr(gz), gz.size, _ptr(pt), pt.shape[0],
                                 _ptr(out)))
+    return out
+
+
+def adjoint_gradient(fwd: "LibSim", adj: "LibSim", fwd_out, adj_out, freq, eps_prime, norm, center, size, dim):
+    """ffip_adjoint_gradient: gradient on the density nodes [nz, ny, nx]."""
+    fo = np.ascontiguousarray(fwd_out, dtype=np.int32)
+    ao = np.ascontiguousarray(adj_out, dtype=np.int32)
+    ep = np.array([complex(eps_prime).real, complex(eps_prime).imag])
+    nm = np.array([complex(norm).real, complex(norm).imag])
+    c = np.array([center[0], center[1], center[2]], dtype=np.float64)
+    s = np.array([size[0], size[1], size[2]], dtype=np.float64)
+    d = np.array([int(round(dim[0])), int(round(dim[1])), int(round(dim[2]))], dtype=np.int64)
+    out = np.zeros((d[2], d[1], d[0]))
+    _check(lib().ffip_adjoint_gradient(fwd._h, adj._h, _ptr(fo), _ptr(ao), float(freq), _ptr(ep), _ptr(nm), _ptr(c),
+                                       _ptr(s), _ptr(d), _ptr(out)))
     return out
 
 

@@ -519,6 +519,19 @@ int ffip_sim_info(const ffip_sim* sim, double out[6])
 
 void ffip_sim_destroy(ffip_sim* sim) { delete sim; }
 
+int ffip_adjoint_gradient(const ffip_sim* fwd, const ffip_sim* adj, const int fwd_out[3], const int adj_out[3],
+                          double freq, const double eps_prime[2], const double norm[2], const double center[3],
+                          const double size[3], const int64_t dim[3], double* grad)
+{
+    if (!fwd_out || !adj_out || !eps_prime || !norm || !center || !size || !dim || !grad)
+        return fail(FFIP_EINVAL, "null argument");
+    return guard([&] {
+        Engine& F = need_engine(const_cast<ffip_sim*>(fwd));
+        Engine& A = need_engine(const_cast<ffip_sim*>(adj));
+        F.adjoint_gradient(A, fwd_out, adj_out, freq, eps_prime, norm, center, size, dim, grad);
+    });
+}
+
 // interpn<N>::transpose with zero padding (include/utility.hpp:702-742, 797-828),
 // driven as in app/ffip.cpp:308-352
 static void transpose_axis(double* data, const int64_t* dims, const size_t* strides, const double* q, int nd,

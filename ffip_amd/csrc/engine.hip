@@ -510,6 +510,90 @@ __global__ void k_point_value(const T* u, const int64_t* idx, const double* w, i
     }
 }
 
+// ---------------------------------------------------------------- adjoint gradient
+// Sensitivity Re(2 pi i f E_fwd E_adj eps' / norm) at the E points strictly inside
+// the design box (ffip/simulation.py:841-869), carried onto the density nodes by
+// the adjoint of zero-padded trilinear interpolation (app/ffip.cpp:308-352).
+// Gather form: one thread per density node sums the points in its support, in a
+// fixed order, so the result is deterministic.
+struct GradComp {
+    const double *fre, *fim, *are, *aim;  // forward / adjoint DFT at the design frequency
+    int lo[3];                            // unit box origin (cell index)
+    int dims[3];                          // unit dims
+    int par[3];                           // parity of the component per axis
+};
+
+struct GradArgs {
+    GradComp c[3];
+    int M[3];
+    double dx2;
+    double blo[3], bhi[3];  // design box, doubled coordinates, strict interior
+    double gx0[3], dxn[3];  // density nodes: origin and spacing (physical)
+    int nd[3];              // density nodes per axis
+    double two_pi_f, epr, epi, nr, ni;
+    double* grad;  // [nz][ny][nx]
+};
+
+__global__ void k_adjoint_gradient(const GradArgs A)
+{
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t nn = int64_t(A.nd[0]) * A.nd[1] * A.nd[2];
+    if (t >= nn)
+        return;
+    const int node[3] = {int(t % A.nd[0]), int((t / A.nd[0]) % A.nd[1]), int(t / (int64_t(A.nd[0]) * A.nd[1]))};
+    // eps' / norm
+    const double den = A.nr * A.nr + A.ni * A.ni;
+    const double qr = (A.epr * A.nr + A.epi * A.ni) / den, qi = (A.epi * A.nr - A.epr * A.ni) / den;
+    double acc = 0;
+    for (int c = 0; c < 3; ++c) {
+        const GradComp& G = A.c[c];
+        int l0[3], l1[3];
+        for (int a = 0; a < 3; ++a) {
+            if (A.nd[a] == 1) {
+                l0[a] = 0;
+                l1[a] = G.dims[a] - 1;
+                continue;
+            }
+            // lattice points whose normalised coordinate lies within one spacing of the node
+            const double xlo = (A.gx0[a] + (node[a] - 1) * A.dxn[a]) / A.dx2;
+            const double xhi = (A.gx0[a] + (node[a] + 1) * A.dxn[a]) / A.dx2;
+            l0[a] = max(0, int(floor((xlo + A.M[a] - 1 + G.par[a]) / 2)) - G.lo[a] - 1);
+            l1[a] = min(G.dims[a] - 1, int(ceil((xhi + A.M[a] - 1 + G.par[a]) / 2)) - G.lo[a] + 1);
+        }
+        for (int kk = l0[2]; kk <= l1[2]; ++kk)
+            for (int jj = l0[1]; jj <= l1[1]; ++jj)
+                for (int ii = l0[0]; ii <= l1[0]; ++ii) {
+                    const int loc[3] = {ii, jj, kk};
+                    double w = 1;
+                    bool ok = true;
+                    for (int a = 0; a < 3 && ok; ++a) {
+                        const double xd = double(2 * (G.lo[a] + loc[a]) - A.M[a] + (1 - G.par[a]));
+                        if (!(xd > A.blo[a] && xd < A.bhi[a])) {
+                            ok = false;
+                            break;
+                        }
+                        if (A.nd[a] == 1)
+                            continue;
+                        const double s = (xd * A.dx2 - A.gx0[a]) / A.dxn[a];
+                        const double d = fabs(s - double(node[a]));
+                        if (!(d < 1)) {
+                            ok = false;
+                            break;
+                        }
+                        w *= 1 - d;
+                    }
+                    if (!ok)
+                        continue;
+                    const int64_t o = ii + int64_t(G.dims[0]) * (jj + int64_t(G.dims[1]) * kk);
+                    const double fr = G.fre[o], fi = G.fim[o], ar = G.are[o], ai = G.aim[o];
+                    const double pr = fr * ar - fi * ai, pi = fr * ai + fi * ar;  // E_fwd E_adj
+                    const double xi = pr * qi + pi * qr;                          // Im(E_fwd E_adj eps'/norm)
+                    acc += w * (-A.two_pi_f * xi);                                // Re(2 pi i f (...))
+                }
+    }
+    A.grad[t] = acc;
+}
+
 // ---------------------------------------------------------------- probes / phasors
 template <typename T>
 struct ProbeArgs {
@@ -1413,6 +1497,82 @@ double Engine::probe(const double pos[3], int ct)
         v += x * t.w;
     }
     return v;
+}
+
+void Engine::adjoint_gradient(const Engine& adj, const int fwd_out[3], const int adj_out[3], double freq,
+                              const double eps_prime[2], const double norm[2], const double center[3],
+                              const double size[3], const int64_t dim[3], double* grad)
+{
+    EngineImpl& F = *p_;
+    const EngineImpl& Ad = *adj.p_;
+    const Setup& S = *F.S;
+    if (Ad.S->L.M[0] != S.L.M[0] || Ad.S->L.n[0] != S.L.n[0] || Ad.S->L.n[1] != S.L.n[1] ||
+        Ad.S->L.n[2] != S.L.n[2])
+        throw std::runtime_error("forward and adjoint simulations have different grids");
+    if (dim[0] < 1 || dim[1] < 1 || dim[2] < 1)
+        throw std::runtime_error("density grid dimension must be positive");
+    GradArgs A;
+    std::memset(&A, 0, sizeof(A));
+    for (int c = 0; c < 3; ++c) {
+        const EngineImpl* E[2] = {&F, &Ad};
+        const int ks[2] = {fwd_out[c], adj_out[c]};
+        const DftUnit* U[2];
+        const DftDev* D[2];
+        int64_t fi[2];
+        for (int s = 0; s < 2; ++s) {
+            const Setup& SS = *E[s]->S;
+            if (ks[s] < 0 || ks[s] >= int(SS.outputs.size()))
+                throw std::runtime_error("DFT output index out of range");
+            auto hu = SS.output_units[ks[s]];
+            U[s] = &SS.units[hu.first][hu.second];
+            D[s] = &E[s]->units[hu.first][hu.second];
+            if (U[s]->comp != c || !U[s]->has_local)
+                throw std::runtime_error("design DFT outputs must be Ex, Ey, Ez over the design box");
+            auto it = std::lower_bound(U[s]->freqs.begin(), U[s]->freqs.end(), freq);
+            if (it == U[s]->freqs.end() || *it != freq)
+                throw std::runtime_error("frequency is not registered");
+            fi[s] = it - U[s]->freqs.begin();
+        }
+        for (int a = 0; a < 3; ++a)
+            if (U[0]->box[a][0] != U[1]->box[a][0] || U[0]->box[a][1] != U[1]->box[a][1])
+                throw std::runtime_error("forward and adjoint design DFTs cover different points");
+        GradComp& G = A.c[c];
+        G.fre = D[0]->re + fi[0] * D[0]->npts;
+        G.fim = D[0]->im + fi[0] * D[0]->npts;
+        G.are = D[1]->re + fi[1] * D[1]->npts;
+        G.aim = D[1]->im + fi[1] * D[1]->npts;
+        for (int a = 0; a < 3; ++a) {
+            G.lo[a] = U[0]->box[a][0];
+            G.dims[a] = U[0]->dims[a];
+            G.par[a] = comp_parity(c, a);
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        A.M[a] = S.L.M[a];
+        const double cd = center[a] / S.dx2, sd = size[a] / S.dx2;  // as Two_Medium_Box (src/geometry.cpp:49-56)
+        A.blo[a] = std::min(cd - sd / 2, cd + sd / 2);
+        A.bhi[a] = std::max(cd - sd / 2, cd + sd / 2);
+        // numpy.linspace(c - s/2, c + s/2, dim) and the spacing transpose() takes from its first two nodes
+        const double g0 = center[a] - size[a] / 2, g1 = center[a] + size[a] / 2;
+        A.gx0[a] = g0;
+        A.dxn[a] = dim[a] > 1 ? (g0 + (g1 - g0) / double(dim[a] - 1)) - g0 : 1.0;
+        A.nd[a] = int(dim[a]);
+    }
+    A.dx2 = S.dx2;
+    A.two_pi_f = 2 * 3.141592653589793e+00 * freq;
+    A.epr = eps_prime[0];
+    A.epi = eps_prime[1];
+    A.nr = norm[0];
+    A.ni = norm[1];
+    const int64_t nn = dim[0] * dim[1] * dim[2];
+    sync();
+    HIP_OK(hipStreamSynchronize(Ad.stream));  // its DFT sums are read on this engine's stream
+    double* g = F.sample_buf(size_t(nn));
+    A.grad = g;
+    k_adjoint_gradient<<<unsigned((nn + 255) / 256), 256, 0, F.stream>>>(A);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(F.stream));
+    HIP_OK(hipMemcpy(grad, g, size_t(nn) * 8, hipMemcpyDeviceToHost));
 }
 
 // Steps in examination windows with one per-step sample kept on the device and

@@ -53,6 +53,14 @@ public:
     void get_component(int ct, double* out);
     void sync();
     double device_bytes() const;
+    // Adjoint-method gradient on the density nodes of a two-medium design box:
+    // Re(2 pi i f E_fwd E_adj eps' / norm) at the E points strictly inside the box
+    // (forward = this engine, adjoint = adj; their design DFT outputs fwd_out /
+    // adj_out are Ex, Ey, Ez), scattered by the adjoint of trilinear interpolation
+    // onto dim = (nx, ny, nz) nodes; grad is [nz][ny][nx] on the host.
+    void adjoint_gradient(const Engine& adj, const int fwd_out[3], const int adj_out[3], double freq,
+                          const double eps_prime[2], const double norm[2], const double center[3],
+                          const double size[3], const int64_t dim[3], double* grad);
     // Algorithmic HBM bytes of one launch of the half-step kernel (h = 0 electric,
     // 1 magnetic): out = {fields, material ids, polarization, CPML psi, DFT sums,
     // total, points updated}.  Each array element the half step must read or write

@@ -113,6 +113,19 @@ int ffip_sim_info(const ffip_sim *sim, double out[6]);
 
 void ffip_sim_destroy(ffip_sim *sim);
 
+/* Adjoint-method gradient of a two-medium design box on the GPU (the computation
+ * of Adjoint_Volume.get_sensitivity, ffip/simulation.py:841-869, with the
+ * transpose of app/ffip.cpp:308-352 done as a deterministic gather):
+ *   grad[nz][ny][nx] = sum over E points p strictly inside the box of
+ *       Re(2 pi i f E_fwd(p) E_adj(p) eps' / norm) * trilinear weight(p, node)
+ * fwd / adj are finished simulations on the same grid; fwd_out / adj_out are
+ * the indices of their Ex, Ey, Ez DFT outputs over the box; eps_prime and norm
+ * are complex (re, im); center / size / dim describe the density grid as
+ * Two_Medium_Box does (dim = nodes along x, y, z). */
+int ffip_adjoint_gradient(const ffip_sim *fwd, const ffip_sim *adj, const int fwd_out[3], const int adj_out[3],
+                          double freq, const double eps_prime[2], const double norm[2], const double center[3],
+                          const double size[3], const int64_t dim[3], double *grad);
+
 /* Adjoint of zero-padded trilinear interpolation (replaces the pybind helper
  * _ffip.transpose, app/ffip.cpp:308-352): scatters pts[i] = (x, y, z, value) onto
  * the regular grid gx[nx] x gy[ny] x gz[nz] (uniform spacing taken from the first
