@@ -56,6 +56,36 @@ def cpu_baseline(cfg, ds, steps, cells):
     return cells * steps / dt / 1e6
 
 
+def design_gradient(steps):
+    """Design gradients per second on the inverse-design workload: 200x200x100
+    cells, 10-cell CPML, Ex sheet at z = -35, a 100x100x20 two-medium design box
+    (eps 1..12, blurred seeded density), F = sum |DFT Ez|^2 over a 20x20 patch at
+    z = +30, f = 0.1, `steps` forward + `steps` adjoint steps.  One gradient is
+    the whole forward solve, the adjoint solve and the device gather, set-ups
+    included; FP32 storage is compared with FP64 storage on the same density."""
+    import numpy as np
+    from ffip_amd import adjoint as A
+    from ffip_amd.workloads import blurred_density
+    out, grads = {}, {}
+    for prec in ("fp64", "fp32"):
+        p = A.c3_problem(steps=steps, precision=prec)
+        rho = blurred_density(p.shape, seed=3, radius=3, sigma=1.5)
+        p.value_and_grad(rho)  # warm-up: first launches, phasor tables
+        t0 = time.perf_counter()
+        F, g = p.value_and_grad(rho)
+        out[prec] = time.perf_counter() - t0
+        grads[prec] = (F, g)
+        print("[bench] design gradient %s: %.3f s (F = %.6e)" % (prec, out[prec], F), file=sys.stderr, flush=True)
+    g64, g32 = grads["fp64"][1], grads["fp32"][1]
+    err = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+    return {"value": round(1.0 / out["fp32"], 4), "unit": "design-gradients/s", "storage": "fp32",
+            "fp64_value": round(1.0 / out["fp64"], 4), "speedup_vs_fp64": round(out["fp64"] / out["fp32"], 3),
+            "rel_err_vs_fp64": err,
+            "objective_rel_err_vs_fp64": abs(grads["fp32"][0] - grads["fp64"][0]) / abs(grads["fp64"][0]),
+            "workload": "inverse design 200x200x100, 100x100x20 design box, eps 1..12, 20x20 Ez patch at f = 0.1, "
+                        "%d + %d steps, set-ups included" % (steps, steps)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +94,8 @@ def main():
     ap.add_argument("--workload", default="dispersive256")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--gradient-steps", type=int, default=3000)
+    ap.add_argument("--no-gradient", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine holds the whole grid on a single MI355X and has no "
@@ -121,6 +153,8 @@ def main():
               "setup_s": round(t_setup, 2)}
     sim.close()
 
+    grad = None if args.no_gradient else design_gradient(args.gradient_steps)
+
     cpu = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         v = cpu_baseline(cfg, ds, args.cpu_steps, cells)
@@ -135,7 +169,7 @@ def main():
             "data": "synthetic",
             "config": {"workload": desc, "grid": [int(s) for s in cfg["size"]], "storage": cfg.get("precision"),
                        "cells": cells},
-            "roofline": roof, "cpu_baseline": cpu, "detail": detail}
+            "roofline": roof, "cpu_baseline": cpu, "design_gradient": grad, "detail": detail}
     print(json.dumps(line), flush=True)
 
 

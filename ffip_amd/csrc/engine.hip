@@ -1,14 +1,16 @@
 // Fused FDTD half-step kernels for gfx950 and the device engine around them.
 //
 // One launch per half step (k_half<IS_E>) does, for every real point of the three
-// components it updates, what the reference does in four passes:
+// components it updates, what the reference does in three passes:
 //   curl + CPML psi      (PML_Stepping::step / Curl_Stepping::step, src/fields.cpp:89-148)
 //   ADE poles + update   (Medium_Stepping::step, src/structure.cpp:56-85)
-//   DFT accumulation     (DFT_Unit::update, src/dft.cpp:29-62)
+//   D/B DFT accumulation (DFT_Unit::update, src/dft.cpp:29-62; these need the
+//                         polarization sums formed in the update)
 // Point sources (Gaussian_Dipoles_Stepping::step, src/fields.cpp:17-33) touch a
 // few points only and run in a small follow-up kernel that also patches the DFT
-// sums of the points it changes; the ghost fills of Period/PEC/PMC boundaries
-// (Fields::sync_boundary, src/fields.cpp:327-482) are a third small kernel.
+// sums of the points it changes; E/H DFT monitors are accumulated by a
+// one-thread-per-point kernel after it; the ghost fills of Period/PEC/PMC
+// boundaries (Fields::sync_boundary, src/fields.cpp:327-482) come last.
 //
 // The update is written in "E form": E^{n+1} = E^n + (dt*curl - (sum P^{n+1} -
 // sum P^n)) / eps_inf.  Algebraically this is the reference's D form
@@ -17,8 +19,11 @@
 // formed from the stored (rounded) P values so that D stays consistent when
 // fields are stored in FP32.  All arithmetic is FP64; T is the storage type.
 //
-// Tiling: a 64x4 workgroup owns an x-y tile and marches a z segment, carrying the
-// two z-neighbour components in registers; x/y neighbours come from L1/L2.
+// Tiling: a BX x BY (32 x 8) workgroup owns an x-y tile and marches a z segment,
+// carrying the two z-neighbour components in registers; x/y neighbours come from
+// L1/L2.  Each batch of planes issues all its loads before computing (CPML
+// auxiliaries and polarizations first, then the loads every thread makes), so a
+// batch costs one memory round trip; material words are fetched a batch ahead.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,11 +46,17 @@ namespace ffip_amd {
             throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #expr); \
     } while (0)
 
-static constexpr int BX = 32;
-static constexpr int BY = 8;
+// workgroup tile: BX consecutive x points (one wavefront row when BX = 64) by BY rows
+#ifndef FFIP_BX
+#define FFIP_BX 32
+#endif
+#ifndef FFIP_BY
+#define FFIP_BY 8
+#endif
+static constexpr int BX = FFIP_BX;
+static constexpr int BY = FFIP_BY;
 static constexpr int MAXU = 16;
 static constexpr int MAXS = 8;  // polarization slots per point
-static constexpr int MAXMAT = 256;
 static constexpr int PH_CHUNK = 1024;
 
 struct DftDev {
@@ -89,22 +100,27 @@ struct HalfArgs {
     const uint32_t* ids;  // per cell: id of component 0 | 1 << 8 | 2 << 16
     const MatEntry* mats;
     int nmats;
-    const double* p_c1;
-    const double* p_c2;
-    const double* p_sc3;
+    const double* coef;  // [npoles][3] = (c1, c2, sigma*c3) of the table's poles
+    int npoles;
     int d_box[3][3][2];
     int d_np[3];
     const double* d_inv[3];
     const double* d_eps[3];
     const double* d_amp[3];
     const double* d_c[3];  // [np][3] = (c1, c2, c3)
-    // polarization state: slot s, component c at (s*3 + c) * pstride
-    T* Pcur;
-    T* Pprev;
-    int64_t pstride;
+    // polarization state of slot s, component c: slice s*3 + c of pcur (read) and of
+    // pprev (read, then overwritten with P^{n+1}); the buffers swap roles every step
+    T* pcur;
+    T* pprev;
+    int64_t pslice;  // elements per slice
+    int origin;  // cell offset of lattice index (0, 0, 0) in every per-cell array
     PmlDev pml[3];
     int nunits;
     DftDev units[MAXU];
+    // the units each component accumulates in this kernel, and the union of their boxes
+    int nfu[3];
+    int fu[3][MAXU];
+    int fbox[3][3][2];
     const double* ph;  // this half step's phasor row [mf][2] = (cos wt, -sin wt)
 };
 
@@ -115,107 +131,254 @@ __device__ __forceinline__ double gaussian1(double t, double width)
     return arg * exp(-arg * arg / 2);
 }
 
+// Column constants of one thread: position, CPML slab positions and
+// coefficients along x and y, and which of the three components live here.
+struct ColCtx {
+    int i, j, base;
+    int sx, sy;  // slab positions (-1 outside the CPML)
+    double bx, cx, ikx, by, cy, iky;
+    bool in[3];
+};
+
+// Plane constants (uniform over the workgroup): CPML along z.
+struct ZCtx {
+    int k, o, sz;
+    double bz, cz, ikz;
+};
+
+// Loads and stores at an unsigned 32-bit byte offset from a workgroup-uniform
+// base: the base stays in scalar registers and loads that share an offset share
+// one vector register (global_load v, v_off, s[base]).
+template <typename T>
+__device__ __forceinline__ T ldb(const T* base, uint32_t off)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ void stb(T* base, uint32_t off, T v)
+{
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+// Everything one thread reads for one plane.  All of it is requested before any
+// of it is used, so a plane costs one memory round trip whatever it holds (CPML
+// auxiliaries, several poles, dense-region coefficients).
+template <typename T, int NPM>
+struct PlaneIn {
+    T n0, n1, v2, v2a, v2b, v1a, v0b;  // stencil operands (see k_half)
+    T e[3];                            // the three updated components
+    T psi[3][2];                       // CPML auxiliaries of the two curl terms of each component
+    T pc[3][NPM > 0 ? NPM : 1];        // P^n of each pole
+    T pp[3][NPM > 0 ? NPM : 1];        // P^{n-1}
+    double dinv[3];                    // 1/eps_inf in the dense region
+};
+
+// CPML auxiliary offsets (slab-only arrays, src/fields.cpp:89-109): the slab along
+// axis a stores [W][n_b][n_c] with the slab position replacing that axis.
+template <typename T>
+__device__ __forceinline__ int psi_off_x(const HalfArgs<T>& A, const ColCtx& X, int k)
+{
+    return X.sx + A.pml[0].W * (X.j + A.n[1] * k);
+}
+template <typename T>
+__device__ __forceinline__ int psi_off_y(const HalfArgs<T>& A, const ColCtx& X, int k)
+{
+    return X.i + A.n[0] * (X.sy + A.pml[1].W * k);
+}
+template <typename T>
+__device__ __forceinline__ int psi_off_z(const HalfArgs<T>& A, const ColCtx& X, int sz)
+{
+    return X.i + A.n[0] * (X.j + A.n[1] * sz);
+}
+
+template <typename T>
+__device__ __forceinline__ int64_t dense_off(const HalfArgs<T>& A, int C, int i, int j, int k)
+{
+    const int b00 = A.d_box[C][0][0], b10 = A.d_box[C][1][0], b20 = A.d_box[C][2][0];
+    const int64_t d0 = A.d_box[C][0][1] - b00 + 1, d1n = A.d_box[C][1][1] - b10 + 1;
+    return (int64_t(k - b20) * d1n + (j - b10)) * d0 + (i - b00);
+}
+
+// Stencil operands and updated components of one plane (loads every thread
+// issues; o is clamped into the segment so they are always in bounds).
+template <typename T, bool IS_E, int NPM>
+__device__ __forceinline__ void load_fields(const HalfArgs<T>& A, int o, PlaneIn<T, NPM>& L)
+{
+    constexpr uint32_t E = sizeof(T);
+    const uint32_t b = uint32_t(o) * E, by = uint32_t(A.pitch) * E, bz = uint32_t(A.plane) * E;
+    if (IS_E) {
+        L.n0 = ldb(A.s[0], b + bz);
+        L.n1 = ldb(A.s[1], b + bz);
+        L.v2a = ldb(A.s[2], b + E);
+        L.v2b = ldb(A.s[2], b + by);
+        L.v1a = ldb(A.s[1], b + E);
+        L.v0b = ldb(A.s[0], b + by);
+    } else {
+        L.n0 = ldb(A.s[0], b);
+        L.n1 = ldb(A.s[1], b);
+        L.v2a = ldb(A.s[2], b - E);
+        L.v2b = ldb(A.s[2], b - by);
+        L.v1a = ldb(A.s[1], b - E);
+        L.v0b = ldb(A.s[0], b - by);
+    }
+    L.v2 = ldb(A.s[2], b);
+#pragma unroll
+    for (int C = 0; C < 3; ++C)
+        L.e[C] = ldb<T>(A.u[C], b);
+}
+
+// Loads that only some points need: CPML auxiliaries, polarizations, dense
+// 1/eps_inf.  A batch issues all of these before its unconditional loads, so
+// that waiting for the (newer) field data also covers them and the wait counts
+// stay exact whichever of these branches ran.  `id` is the packed material
+// word of the cell, fetched one batch earlier.
+template <typename T, bool MAT, int NPM>
+__device__ __forceinline__ void load_sparse(const HalfArgs<T>& A, const MatEntry* smat, const ColCtx& X,
+                                            const ZCtx& Z, uint32_t id, const bool act[3], PlaneIn<T, NPM>& L)
+{
+    const int o = Z.o;
+    // CPML auxiliaries: component 0 differentiates along (y, z), 1 along (z, x), 2 along (x, y)
+    if (act[0]) {
+        if (X.sy >= 0)
+            L.psi[0][0] = ldb(static_cast<const T*>(A.pml[1].psi1), uint32_t(psi_off_y(A, X, Z.k)) * sizeof(T));
+        if (Z.sz >= 0)
+            L.psi[0][1] = ldb(static_cast<const T*>(A.pml[2].psi2), uint32_t(psi_off_z(A, X, Z.sz)) * sizeof(T));
+    }
+    if (act[1]) {
+        if (Z.sz >= 0)
+            L.psi[1][0] = ldb(static_cast<const T*>(A.pml[2].psi1), uint32_t(psi_off_z(A, X, Z.sz)) * sizeof(T));
+        if (X.sx >= 0)
+            L.psi[1][1] = ldb(static_cast<const T*>(A.pml[0].psi2), uint32_t(psi_off_x(A, X, Z.k)) * sizeof(T));
+    }
+    if (act[2]) {
+        if (X.sx >= 0)
+            L.psi[2][0] = ldb(static_cast<const T*>(A.pml[0].psi1), uint32_t(psi_off_x(A, X, Z.k)) * sizeof(T));
+        if (X.sy >= 0)
+            L.psi[2][1] = ldb(static_cast<const T*>(A.pml[1].psi2), uint32_t(psi_off_y(A, X, Z.k)) * sizeof(T));
+    }
+    if (MAT) {
+#pragma unroll
+        for (int C = 0; C < 3; ++C) {
+            if (!act[C])
+                continue;
+            const int mid = int((id >> (8 * C)) & 255u);
+            int np;
+            if (mid != 255) {
+                np = smat[mid].np;
+            } else {
+                np = A.d_np[C];
+                L.dinv[C] = A.d_inv[C][dense_off(A, C, X.i, X.j, Z.k)];
+            }
+#pragma unroll
+            for (int s = 0; s < NPM; ++s) {
+                if (s < np) {
+                    L.pc[C][s] = ldb<T>(A.pcur + (s * 3 + C) * A.pslice, uint32_t(o) * sizeof(T));
+                    L.pp[C][s] = ldb<T>(A.pprev + (s * 3 + C) * A.pslice, uint32_t(o) * sizeof(T));
+                }
+            }
+        }
+    }
+}
+
 // one CPML term (src/fields.cpp:89-109): psi <- b psi + c d; returns psi + d / kappa
 template <typename T>
-__device__ __forceinline__ double cpml_term(T* psi, int q, double b, double c, double ik, double d)
+__device__ __forceinline__ double cpml_apply(void* arr, int q, T psi, double b, double c, double ik, double d)
 {
-    const double ps = b * double(psi[q]) + c * d;
-    psi[q] = T(ps);
+    const double ps = b * double(psi) + c * d;
+    stb(static_cast<T*>(arr), uint32_t(q) * sizeof(T), T(ps));
     return ps + d * ik;
 }
 
-struct PlaneCtx {
-    int i, j, k, o;
-    // slab positions and coefficients of the three axes at this point
-    int sx, sy, sz;
-    double bx, cx, ikx, by, cy, iky, bz, cz, ikz;
-};
-
-template <typename T, bool MAT, int C>
-__device__ __forceinline__ void update_point(const HalfArgs<T>& A, const MatEntry* smat, const PlaneCtx& P,
-                                             double d1, double d2, T e_in, uint32_t idword)
+// Update component C of one point from its preloaded plane data: CPML, ADE
+// poles (src/structure.cpp:56-85, in E form), DFT accumulation, store.
+template <typename T, bool MAT, int NPM, int C>
+__device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry* smat, const double* spc,
+                                            const ColCtx& X, const ZCtx& Z, double d1, double d2, uint32_t id,
+                                            const PlaneIn<T, NPM>& L)
 {
     // term 1 differentiates along a1 = (C+1)%3, term 2 along a2 = (C+2)%3
     double t1 = d1, t2 = d2;
-    if (C == 0) {  // a1 = y, a2 = z
-        if (P.sy >= 0)
-            t1 = cpml_term<T>(static_cast<T*>(A.pml[1].psi1), P.i + A.n[0] * (P.sy + A.pml[1].W * P.k), P.by, P.cy,
-                              P.iky, d1);
-        if (P.sz >= 0)
-            t2 = cpml_term<T>(static_cast<T*>(A.pml[2].psi2), P.i + A.n[0] * (P.j + A.n[1] * P.sz), P.bz, P.cz,
-                              P.ikz, d2);
-    } else if (C == 1) {  // a1 = z, a2 = x
-        if (P.sz >= 0)
-            t1 = cpml_term<T>(static_cast<T*>(A.pml[2].psi1), P.i + A.n[0] * (P.j + A.n[1] * P.sz), P.bz, P.cz,
-                              P.ikz, d1);
-        if (P.sx >= 0)
-            t2 = cpml_term<T>(static_cast<T*>(A.pml[0].psi2), P.sx + A.pml[0].W * (P.j + A.n[1] * P.k), P.bx, P.cx,
-                              P.ikx, d2);
-    } else {  // a1 = x, a2 = y
-        if (P.sx >= 0)
-            t1 = cpml_term<T>(static_cast<T*>(A.pml[0].psi1), P.sx + A.pml[0].W * (P.j + A.n[1] * P.k), P.bx, P.cx,
-                              P.ikx, d1);
-        if (P.sy >= 0)
-            t2 = cpml_term<T>(static_cast<T*>(A.pml[1].psi2), P.i + A.n[0] * (P.sy + A.pml[1].W * P.k), P.by, P.cy,
-                              P.iky, d2);
+    if (C == 0) {
+        if (X.sy >= 0)
+            t1 = cpml_apply<T>(A.pml[1].psi1, psi_off_y(A, X, Z.k), L.psi[0][0], X.by, X.cy, X.iky, d1);
+        if (Z.sz >= 0)
+            t2 = cpml_apply<T>(A.pml[2].psi2, psi_off_z(A, X, Z.sz), L.psi[0][1], Z.bz, Z.cz, Z.ikz, d2);
+    } else if (C == 1) {
+        if (Z.sz >= 0)
+            t1 = cpml_apply<T>(A.pml[2].psi1, psi_off_z(A, X, Z.sz), L.psi[1][0], Z.bz, Z.cz, Z.ikz, d1);
+        if (X.sx >= 0)
+            t2 = cpml_apply<T>(A.pml[0].psi2, psi_off_x(A, X, Z.k), L.psi[1][1], X.bx, X.cx, X.ikx, d2);
+    } else {
+        if (X.sx >= 0)
+            t1 = cpml_apply<T>(A.pml[0].psi1, psi_off_x(A, X, Z.k), L.psi[2][0], X.bx, X.cx, X.ikx, d1);
+        if (X.sy >= 0)
+            t2 = cpml_apply<T>(A.pml[1].psi2, psi_off_y(A, X, Z.k), L.psi[2][1], X.by, X.cy, X.iky, d2);
     }
     const double acc = t1 - t2;  // increment of D/dt (B/dt)
-    const double e_old = double(e_in);
-    double e_new, eps, sumP = 0;
+    const double e_old = double(L.e[C]);
+    double e_new, sumP = 0;
+    int mid = 0;
     if (!MAT) {
         e_new = e_old + A.dt * acc * A.u_inv;
-        eps = A.u_eps;
     } else {
-        const int id = int((idword >> (8 * C)) & 255u);
+        mid = int((id >> (8 * C)) & 255u);
         double inv, dP = 0;
-        if (id != 255) {
-            const MatEntry m = smat[id];
+        if (mid != 255) {
+            const MatEntry m = smat[mid];
             inv = m.inv;
-            eps = m.eps;
-            for (int s = 0; s < m.np; ++s) {
-                const int64_t o = (int64_t(s) * 3 + C) * A.pstride + P.o;
-                const double pc = double(A.Pcur[o]);
-                const double pp = double(A.Pprev[o]);
-                const double pn = A.p_c1[m.p0 + s] * pc + A.p_c2[m.p0 + s] * pp + A.p_sc3[m.p0 + s] * e_old;
-                const T pt = T(pn);
-                A.Pprev[o] = pt;
-                dP += double(pt) - pc;
-                sumP += double(pt);
+#pragma unroll
+            for (int s = 0; s < NPM; ++s) {
+                if (s < m.np) {
+                    const double* cf = spc + 3 * (m.p0 + s);
+                    const double pc = double(L.pc[C][s]);
+                    const double pn = cf[0] * pc + cf[1] * double(L.pp[C][s]) + cf[2] * e_old;
+                    const T pt = T(pn);
+                    stb<T>(A.pprev + (s * 3 + C) * A.pslice, uint32_t(Z.o) * sizeof(T), pt);
+                    dP += double(pt) - pc;
+                    sumP += double(pt);
+                }
             }
         } else {
-            const int b00 = A.d_box[C][0][0], b10 = A.d_box[C][1][0], b20 = A.d_box[C][2][0];
-            const int64_t d0 = A.d_box[C][0][1] - b00 + 1, d1n = A.d_box[C][1][1] - b10 + 1;
-            const int64_t q = (int64_t(P.k - b20) * d1n + (P.j - b10)) * d0 + (P.i - b00);
-            inv = A.d_inv[C][q];
-            eps = A.d_eps[C][q];
+            inv = L.dinv[C];
             const int np = A.d_np[C];
-            for (int s = 0; s < np; ++s) {
-                const int64_t o = (int64_t(s) * 3 + C) * A.pstride + P.o;
-                const double pc = double(A.Pcur[o]);
-                const double pp = double(A.Pprev[o]);
-                const double amp = A.d_amp[C][q * np + s];
-                const double* cc = A.d_c[C] + 3 * s;
-                const double pn = cc[0] * pc + cc[1] * pp + amp * cc[2] * e_old;
-                const T pt = T(pn);
-                A.Pprev[o] = pt;
-                dP += double(pt) - pc;
-                sumP += double(pt);
+            if (NPM > 0 && np > 0) {
+                const int64_t q = dense_off(A, C, X.i, X.j, Z.k);
+#pragma unroll
+                for (int s = 0; s < NPM; ++s) {
+                    if (s < np) {
+                        const double* cc = A.d_c[C] + 3 * s;
+                        const double amp = A.d_amp[C][q * np + s];
+                        const double pc = double(L.pc[C][s]);
+                        const double pn = cc[0] * pc + cc[1] * double(L.pp[C][s]) + amp * cc[2] * e_old;
+                        const T pt = T(pn);
+                        stb<T>(A.pprev + (s * 3 + C) * A.pslice, uint32_t(Z.o) * sizeof(T), pt);
+                        dP += double(pt) - pc;
+                        sumP += double(pt);
+                    }
+                }
             }
         }
         e_new = e_old + (A.dt * acc - dP) * inv;
     }
     const T stored = T(e_new);
-    A.u[C][P.o] = stored;
     // DFT monitors (k is uniform across the workgroup)
-    for (int m = 0; m < A.nunits; ++m) {
-        const DftDev& U = A.units[m];
-        if (!U.fused || U.comp != C || P.k < U.box[2][0] || P.k > U.box[2][1])
+    const bool in_dft = A.nfu[C] > 0 && Z.k >= A.fbox[C][2][0] && Z.k <= A.fbox[C][2][1] &&
+                        X.i >= A.fbox[C][0][0] && X.i <= A.fbox[C][0][1] && X.j >= A.fbox[C][1][0] &&
+                        X.j <= A.fbox[C][1][1];
+    for (int m = 0; in_dft && m < A.nfu[C]; ++m) {
+        const DftDev& U = A.units[A.fu[C][m]];
+        if (Z.k < U.box[2][0] || Z.k > U.box[2][1])
             continue;
-        if (P.i < U.box[0][0] || P.i > U.box[0][1] || P.j < U.box[1][0] || P.j > U.box[1][1])
+        if (X.i < U.box[0][0] || X.i > U.box[0][1] || X.j < U.box[1][0] || X.j > U.box[1][1])
             continue;
-        const double v = U.is_db ? eps * double(stored) + sumP : double(stored);
-        const int64_t off = (P.i - U.box[0][0]) +
-                            int64_t(U.dims[0]) * ((P.j - U.box[1][0]) + int64_t(U.dims[1]) * (P.k - U.box[2][0]));
+        double v = double(stored);
+        if (U.is_db) {
+            const double eps = !MAT ? A.u_eps
+                                    : (mid != 255 ? smat[mid].eps : A.d_eps[C][dense_off(A, C, X.i, X.j, Z.k)]);
+            v = eps * double(stored) + sumP;
+        }
+        const int64_t off = (X.i - U.box[0][0]) +
+                            int64_t(U.dims[0]) * ((X.j - U.box[1][0]) + int64_t(U.dims[1]) * (Z.k - U.box[2][0]));
         const double* ph = A.ph + 2 * U.mf0;
         for (int f = 0; f < U.nf; ++f) {
             const int64_t o = int64_t(f) * U.npts + off;
@@ -223,19 +386,28 @@ __device__ __forceinline__ void update_point(const HalfArgs<T>& A, const MatEntr
             U.re[o] += ph[2 * f] * v;
         }
     }
+    stb<T>(A.u[C], uint32_t(Z.o) * sizeof(T), stored);
 }
 
 // One half step over a (BX x BY) column tile and a z segment of A.lz planes.
 // Loads for U consecutive planes are issued before any of them is computed so
 // that each thread keeps several planes of requests in flight; the two
 // components differentiated along z are carried between planes in registers.
-template <typename T, bool IS_E, bool MAT, int U>
+template <typename T, bool IS_E, bool MAT, int NPM, int U>
 __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
 {
-    __shared__ MatEntry smat[MAT ? MAXMAT : 1];
+    // material table and pole coefficients, staged in LDS (sized at launch:
+    // 24 bytes per table entry and per pole)
+    extern __shared__ double lds_tab[];
+    MatEntry* smat = reinterpret_cast<MatEntry*>(lds_tab);
+    double* spc = lds_tab + 3 * A.nmats;
     if (MAT) {
-        for (int q = int(threadIdx.x + BX * threadIdx.y); q < A.nmats; q += BX * BY)
+        const int tid = int(threadIdx.x + BX * threadIdx.y);
+        for (int q = tid; q < A.nmats; q += BX * BY)
             smat[q] = A.mats[q];
+        if (NPM > 0)
+            for (int q = tid; q < 3 * A.npoles; q += BX * BY)
+                spc[q] = A.coef[q];
         __syncthreads();
     }
     // XCD-aware order: consecutive dispatches land on the 8 XCDs in turn, so the
@@ -254,103 +426,103 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
     const int ke = min(kb + A.lz - 1, A.dhi[2]);
     if (i > A.dhi[0] || j > A.dhi[1])
         return;
-    const int sy = A.pitch, sz = A.plane;
+    const int sz = A.plane;
     const double inv_dx = A.inv_dx;
-    const T* __restrict__ s0 = A.s[0];
-    const T* __restrict__ s1 = A.s[1];
-    const T* __restrict__ s2 = A.s[2];
-    const T* __restrict__ u0 = A.u[0];
-    const T* __restrict__ u1 = A.u[1];
-    const T* __restrict__ u2 = A.u[2];
-    const bool in0 = i >= A.lo[0][0] && i <= A.hi[0][0] && j >= A.lo[0][1] && j <= A.hi[0][1];
-    const bool in1 = i >= A.lo[1][0] && i <= A.hi[1][0] && j >= A.lo[1][1] && j <= A.hi[1][1];
-    const bool in2 = i >= A.lo[2][0] && i <= A.hi[2][0] && j >= A.lo[2][1] && j <= A.hi[2][1];
-    PlaneCtx P;
-    P.i = i;
-    P.j = j;
-    P.sx = A.pml[0].W ? A.pml[0].smap[i] : -1;
-    P.sy = A.pml[1].W ? A.pml[1].smap[j] : -1;
-    P.bx = P.cx = P.by = P.cy = P.bz = P.cz = 0;
-    P.ikx = P.iky = P.ikz = 1;
-    if (P.sx >= 0) {
-        P.bx = A.pml[0].b[i];
-        P.cx = A.pml[0].c[i];
-        P.ikx = A.pml[0].ik[i];
+    ColCtx X;
+    X.i = i;
+    X.j = j;
+    X.base = A.origin + i + A.pitch * j;
+#pragma unroll
+    for (int C = 0; C < 3; ++C)
+        X.in[C] = i >= A.lo[C][0] && i <= A.hi[C][0] && j >= A.lo[C][1] && j <= A.hi[C][1];
+    X.sx = A.pml[0].W ? A.pml[0].smap[i] : -1;
+    X.sy = A.pml[1].W ? A.pml[1].smap[j] : -1;
+    X.bx = X.cx = X.by = X.cy = 0;
+    X.ikx = X.iky = 1;
+    if (X.sx >= 0) {
+        X.bx = A.pml[0].b[i];
+        X.cx = A.pml[0].c[i];
+        X.ikx = A.pml[0].ik[i];
     }
-    if (P.sy >= 0) {
-        P.by = A.pml[1].b[j];
-        P.cy = A.pml[1].c[j];
-        P.iky = A.pml[1].ik[j];
+    if (X.sy >= 0) {
+        X.by = A.pml[1].b[j];
+        X.cy = A.pml[1].c[j];
+        X.iky = A.pml[1].ik[j];
     }
-    const int base = i + sy * j;
     T c0, c1;  // E: s0, s1 at plane k; H: at plane k-1
     {
-        const int o = base + sz * kb;
-        c0 = IS_E ? s0[o] : s0[o - sz];
-        c1 = IS_E ? s1[o] : s1[o - sz];
+        const int o = X.base + sz * kb;
+        const uint32_t b = uint32_t(IS_E ? o : o - sz) * sizeof(T);
+        c0 = ldb(A.s[0], b);
+        c1 = ldb(A.s[1], b);
     }
-    for (int kc = kb; kc <= ke; kc += U) {
-        T n0[U], n1[U], v2[U], v2a[U], v2b[U], v1a[U], v0b[U], e0[U], e1[U], e2[U];
-        uint32_t idv[U];
+    // material words are fetched one batch ahead of the planes they describe
+    uint32_t idn[U];
 #pragma unroll
-        for (int q = 0; q < U; ++q) {
-            const int o = base + sz * (kc + q);
-            if (kc + q <= ke) {
-                if (IS_E) {
-                    n0[q] = s0[o + sz];
-                    n1[q] = s1[o + sz];
-                    v2a[q] = s2[o + 1];
-                    v2b[q] = s2[o + sy];
-                    v1a[q] = s1[o + 1];
-                    v0b[q] = s0[o + sy];
-                } else {
-                    n0[q] = s0[o];
-                    n1[q] = s1[o];
-                    v2a[q] = s2[o - 1];
-                    v2b[q] = s2[o - sy];
-                    v1a[q] = s1[o - 1];
-                    v0b[q] = s0[o - sy];
-                }
-                v2[q] = s2[o];
-                e0[q] = u0[o];
-                e1[q] = u1[o];
-                e2[q] = u2[o];
-                idv[q] = MAT ? A.ids[o] : 0u;
-            }
-        }
+    for (int q = 0; q < U; ++q)
+        idn[q] = MAT ? ldb(A.ids, uint32_t(X.base + sz * min(kb + q, ke)) * 4u) : 0u;
+    for (int kc = kb; kc <= ke; kc += U) {
+        uint32_t idc[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+            idc[q] = idn[q];
+        PlaneIn<T, NPM> L[U];
+        ZCtx Z[U];
+        bool act[U][3];
 #pragma unroll
         for (int q = 0; q < U; ++q) {
             const int k = kc + q;
-            if (k > ke)
+            Z[q].k = k;
+            Z[q].o = X.base + sz * k;
+            Z[q].sz = -1;
+            Z[q].bz = Z[q].cz = 0;
+            Z[q].ikz = 1;
+            if (k <= ke && A.pml[2].W) {
+                Z[q].sz = A.pml[2].smap[k];
+                if (Z[q].sz >= 0) {
+                    Z[q].bz = A.pml[2].b[k];
+                    Z[q].cz = A.pml[2].c[k];
+                    Z[q].ikz = A.pml[2].ik[k];
+                }
+            }
+#pragma unroll
+            for (int C = 0; C < 3; ++C)
+                act[q][C] = k <= ke && X.in[C] && k >= A.lo[C][2] && k <= A.hi[C][2];
+            if (k <= ke)
+                load_sparse<T, MAT, NPM>(A, smat, X, Z[q], idc[q], act[q], L[q]);
+        }
+        // then the loads every thread makes: next batch's material words, fields
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+            idn[q] = MAT ? ldb(A.ids, uint32_t(X.base + sz * min(kc + U + q, ke)) * 4u) : 0u;
+#pragma unroll
+        for (int q = 0; q < U; ++q)
+            load_fields<T, IS_E, NPM>(A, X.base + sz * min(kc + q, ke), L[q]);
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            if (kc + q > ke)
                 break;
-            P.k = k;
-            P.o = base + sz * k;
-            P.sz = A.pml[2].W ? A.pml[2].smap[k] : -1;
-            if (P.sz >= 0) {
-                P.bz = A.pml[2].b[k];
-                P.cz = A.pml[2].c[k];
-                P.ikz = A.pml[2].ik[k];
-            }
+            const PlaneIn<T, NPM>& P = L[q];
             // E: s0(k)=c0, s0(k+1)=n0; H: s0(k-1)=c0, s0(k)=n0 (s1 likewise)
-            const double a0 = double(c0), a1 = double(c1), g0 = double(n0[q]), g1 = double(n1[q]);
-            const double w2 = double(v2[q]);
-            if (in0 && k >= A.lo[0][2] && k <= A.hi[0][2]) {
-                const double d1 = (double(v2b[q]) - w2) * inv_dx;
+            const double a0 = double(c0), a1 = double(c1), g0 = double(P.n0), g1 = double(P.n1);
+            const double w2 = double(P.v2);
+            if (act[q][0]) {
+                const double d1 = (double(P.v2b) - w2) * inv_dx;
                 const double d2 = (IS_E ? (g1 - a1) : (a1 - g1)) * inv_dx;
-                update_point<T, MAT, 0>(A, smat, P, d1, d2, e0[q], idv[q]);
+                update_comp<T, MAT, NPM, 0>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
-            if (in1 && k >= A.lo[1][2] && k <= A.hi[1][2]) {
+            if (act[q][1]) {
                 const double d1 = (IS_E ? (g0 - a0) : (a0 - g0)) * inv_dx;
-                const double d2 = (double(v2a[q]) - w2) * inv_dx;
-                update_point<T, MAT, 1>(A, smat, P, d1, d2, e1[q], idv[q]);
+                const double d2 = (double(P.v2a) - w2) * inv_dx;
+                update_comp<T, MAT, NPM, 1>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
-            if (in2 && k >= A.lo[2][2] && k <= A.hi[2][2]) {
-                const double d1 = (double(v1a[q]) - (IS_E ? a1 : g1)) * inv_dx;
-                const double d2 = (double(v0b[q]) - (IS_E ? a0 : g0)) * inv_dx;
-                update_point<T, MAT, 2>(A, smat, P, d1, d2, e2[q], idv[q]);
+            if (act[q][2]) {
+                const double d1 = (double(P.v1a) - (IS_E ? a1 : g1)) * inv_dx;
+                const double d2 = (double(P.v0b) - (IS_E ? a0 : g0)) * inv_dx;
+                update_comp<T, MAT, NPM, 2>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
-            c0 = n0[q];
-            c1 = n1[q];
+            c0 = P.n0;
+            c1 = P.n1;
         }
     }
 }
@@ -368,22 +540,38 @@ struct SparseDftArgs {
 };
 
 template <typename T>
-__global__ void k_dft_sparse(const SparseDftArgs<T> A)
+__global__ void __launch_bounds__(256) k_dft_sparse(const SparseDftArgs<T> A)
 {
     const DftDev& U = A.units[blockIdx.y];
-    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= U.npts)
+    const int t = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);  // unit sizes are checked < 2^31
+    if (t >= int(U.npts))
         return;
-    const int ii = int(t % U.dims[0]);
-    const int jj = int((t / U.dims[0]) % U.dims[1]);
-    const int kk = int(t / (int64_t(U.dims[0]) * U.dims[1]));
+    const int ii = t % U.dims[0];
+    const int r = t / U.dims[0];
+    const int jj = r % U.dims[1];
+    const int kk = r / U.dims[1];
     const int o = (U.box[0][0] + ii) + A.pitch * (U.box[1][0] + jj) + A.plane * (U.box[2][0] + kk);
     const double v = double(A.u[U.comp][o]);
     const double* ph = A.ph + 2 * U.mf0;
-    for (int f = 0; f < U.nf; ++f) {
-        const int64_t q = int64_t(f) * U.npts + t;
-        U.im[q] += ph[2 * f + 1] * v;
-        U.re[q] += ph[2 * f] * v;
+    double* __restrict__ re = U.re + t;
+    double* __restrict__ im = U.im + t;
+    const int64_t stride = U.npts;
+    // frequencies in chunks: all sums of a chunk are read before any is written
+    constexpr int FC = 8;
+    for (int f0 = 0; f0 < U.nf; f0 += FC) {
+        double a[FC], b[FC];
+#pragma unroll
+        for (int q = 0; q < FC; ++q)
+            if (f0 + q < U.nf) {
+                a[q] = re[(f0 + q) * stride];
+                b[q] = im[(f0 + q) * stride];
+            }
+#pragma unroll
+        for (int q = 0; q < FC; ++q)
+            if (f0 + q < U.nf) {
+                re[(f0 + q) * stride] = a[q] + ph[2 * (f0 + q)] * v;
+                im[(f0 + q) * stride] = b[q] + ph[2 * (f0 + q) + 1] * v;
+            }
     }
 }
 
@@ -654,7 +842,8 @@ struct EngineImpl {
     uint32_t* ids[2] = {};  // packed ids of the three components of a half
     MatEntry* mats[2] = {};
     int nmats[2] = {0, 0};
-    double *p_c1[2] = {}, *p_c2[2] = {}, *p_sc3[2] = {};
+    double* p_coef[2] = {};
+    int npoles[2] = {0, 0};
     int lz = 32;
     double *d_inv[2][3] = {}, *d_eps[2][3] = {}, *d_amp[2][3] = {}, *d_c[2][3] = {};
     PmlDev pml[2][3] = {};
@@ -733,16 +922,39 @@ struct EngineImpl {
     HalfArgs<T> half_args(int h);
     template <typename T>
     void launch_half(int h, hipEvent_t e0, hipEvent_t e1);
-    template <typename T, bool IS_E, bool MAT>
-    void launch_k(int U, dim3 grid, dim3 block, const HalfArgs<T>& A)
+    // npm: most poles at any point of the half (register arrays are sized by it);
+    // U: planes whose loads are issued together
+    template <typename T, bool IS_E>
+    void launch_k(bool mat, int npm, int U, size_t lds, dim3 grid, dim3 block, const HalfArgs<T>& A)
     {
-        switch (U) {
-        case 1: k_half<T, IS_E, MAT, 1><<<grid, block, 0, stream>>>(A); break;
-        case 2: k_half<T, IS_E, MAT, 2><<<grid, block, 0, stream>>>(A); break;
-        default: k_half<T, IS_E, MAT, 4><<<grid, block, 0, stream>>>(A); break;
+        if (!mat) {
+            switch (U) {
+            case 1: k_half<T, IS_E, false, 0, 1><<<grid, block, 0, stream>>>(A); break;
+            case 2: k_half<T, IS_E, false, 0, 2><<<grid, block, 0, stream>>>(A); break;
+            default: k_half<T, IS_E, false, 0, 4><<<grid, block, 0, stream>>>(A); break;
+            }
+            return;
+        }
+        if (U >= 2 && npm <= 4) {
+            switch (npm) {
+            case 0: k_half<T, IS_E, true, 0, 2><<<grid, block, lds, stream>>>(A); break;
+            case 1: k_half<T, IS_E, true, 1, 2><<<grid, block, lds, stream>>>(A); break;
+            case 2: k_half<T, IS_E, true, 2, 2><<<grid, block, lds, stream>>>(A); break;
+            case 3: k_half<T, IS_E, true, 3, 2><<<grid, block, lds, stream>>>(A); break;
+            default: k_half<T, IS_E, true, 4, 2><<<grid, block, lds, stream>>>(A); break;
+            }
+            return;
+        }
+        switch (npm) {
+        case 0: k_half<T, IS_E, true, 0, 1><<<grid, block, lds, stream>>>(A); break;
+        case 1: k_half<T, IS_E, true, 1, 1><<<grid, block, lds, stream>>>(A); break;
+        case 2: k_half<T, IS_E, true, 2, 1><<<grid, block, lds, stream>>>(A); break;
+        case 3: k_half<T, IS_E, true, 3, 1><<<grid, block, lds, stream>>>(A); break;
+        case 4: k_half<T, IS_E, true, 4, 1><<<grid, block, lds, stream>>>(A); break;
+        default: k_half<T, IS_E, true, MAXS, 1><<<grid, block, lds, stream>>>(A); break;
         }
     }
-    int unroll = 4, unroll_mat = 4;  // z planes whose loads are issued together
+    int unroll = 4, unroll_mat = 1;  // z planes whose loads are issued together
     void ensure_phasors();
     template <typename T>
     void step_once(hipEvent_t* ev, bool timed_e, bool timed_h);
@@ -766,12 +978,12 @@ void EngineImpl::build()
     const Layout& L = s.L;
     fp32 = s.precision == "fp32";
     tsz = fp32 ? 4 : 8;
-    // measured on MI355X (256^3 dispersive workload): short z segments give more
-    // workgroups in flight; the material kernel prefers one plane per batch
-    // (fewer live registers, higher occupancy)
+    // measured on MI355X (256^3 dispersive and 200x200x100 design workloads, FP32
+    // and FP64): 8-plane z segments and two planes per load batch; kernels whose
+    // points carry more than four poles fall back to one plane per batch
     lz = 8;
-    unroll = fp32 ? 4 : 2;
-    unroll_mat = 1;
+    unroll = 2;
+    unroll_mat = 2;
     // tuning overrides: z segment per workgroup, planes per load batch
     if (const char* e = std::getenv("FFIP_LZ"))
         lz = std::max(1, std::atoi(e));
@@ -779,8 +991,10 @@ void EngineImpl::build()
         unroll = std::atoi(e);
     if (const char* e = std::getenv("FFIP_UNROLL_MAT"))
         unroll_mat = std::atoi(e);
-    if (L.alloc >= (int64_t(1) << 31))
-        throw std::runtime_error("grid too large for 32-bit cell offsets");
+    // the half-step kernel addresses every per-cell array with a 32-bit byte offset
+    if (L.alloc * int64_t(tsz) >= (int64_t(1) << 32))
+        throw std::runtime_error("grid too large: one field array must stay below 4 GiB (" +
+                                 std::to_string(L.alloc) + " cells)");
     HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (int c = 0; c < 6; ++c) {
         field[c] = fp32 ? (void*)dalloc<float>(L.alloc) : (void*)dalloc<double>(L.alloc);
@@ -806,9 +1020,16 @@ void EngineImpl::build()
                 me[e] = {M.table.inv_eps[e], M.table.eps[e], M.table.npoles[e], M.table.pole0[e]};
             nmats[h] = int(me.size());
             mats[h] = upload(me);
-            p_c1[h] = upload(M.table.c1);
-            p_c2[h] = upload(M.table.c2);
-            p_sc3[h] = upload(M.table.sc3);
+            std::vector<double> cf;
+            for (size_t p = 0; p < M.table.c1.size(); ++p) {
+                cf.push_back(M.table.c1[p]);
+                cf.push_back(M.table.c2[p]);
+                cf.push_back(M.table.sc3[p]);
+            }
+            npoles[h] = int(M.table.c1.size());
+            p_coef[h] = upload(cf);
+            if (size_t(nmats[h] + npoles[h]) * 3 * sizeof(double) > 65536)
+                throw std::runtime_error("material table too large for LDS");
             for (int c = 0; c < 3; ++c) {
                 DenseRegion& D = M.dense[c];
                 if (!D.present)
@@ -869,10 +1090,14 @@ void EngineImpl::build()
                 }
                 d.npts = 0;
             }
-            // rows shorter than 16 points (e.g. planes normal to x) go to the
-            // separate one-thread-per-point kernel; D/B samples need the
-            // polarization sums, so those always stay in the half-step kernel
-            d.fused = (d.is_db || d.dims[0] >= 16) ? 1 : 0;
+            // E/H samples are accumulated by the one-thread-per-point kernel after
+            // the half step: inside the z-marching kernel each monitored plane
+            // would add a dependent read-modify-write round trip per component,
+            // while the separate pass re-reads only the sampled value.  D/B
+            // samples need the polarization sums, so those stay in the half step.
+            d.fused = d.is_db ? 1 : 0;
+            if (d.npts >= (int64_t(1) << 31))
+                throw std::runtime_error("DFT monitor with 2^31 or more points");
             d.re = dalloc<double>(size_t(d.npts) * d.nf);
             d.im = dalloc<double>(size_t(d.npts) * d.nf);
             units[h].push_back(d);
@@ -1057,8 +1282,8 @@ HalfArgs<T> EngineImpl::half_args(int h)
     HalfArgs<T> A;
     std::memset(&A, 0, sizeof(A));
     for (int c = 0; c < 3; ++c) {
-        A.u[c] = static_cast<T*>(field[h * 3 + c]) + L.origin;
-        A.s[c] = static_cast<const T*>(field[(1 - h) * 3 + c]) + L.origin;
+        A.u[c] = static_cast<T*>(field[h * 3 + c]);
+        A.s[c] = static_cast<const T*>(field[(1 - h) * 3 + c]);
     }
     A.pitch = int(L.pitch);
     A.plane = int(L.plane);
@@ -1082,12 +1307,11 @@ HalfArgs<T> EngineImpl::half_args(int h)
     A.u_inv = M.u_inv;
     A.u_eps = M.u_eps;
     if (!M.uniform) {
-        A.ids = ids[h] + L.origin;
+        A.ids = ids[h];
         A.mats = mats[h];
         A.nmats = nmats[h];
-        A.p_c1 = p_c1[h];
-        A.p_c2 = p_c2[h];
-        A.p_sc3 = p_sc3[h];
+        A.coef = p_coef[h];
+        A.npoles = npoles[h];
         for (int c = 0; c < 3; ++c) {
             const DenseRegion& D = M.dense[c];
             if (!D.present)
@@ -1103,16 +1327,30 @@ HalfArgs<T> EngineImpl::half_args(int h)
             A.d_c[c] = d_c[h][c];
         }
     }
-    A.pstride = L.alloc;
-    if (S_slots[h] > 0) {
-        A.Pcur = static_cast<T*>(Pa[h]) + L.origin;
-        A.Pprev = static_cast<T*>(Pb[h]) + L.origin;
-    }
+    A.origin = int(L.origin);
+    A.pcur = static_cast<T*>(Pa[h]);
+    A.pprev = static_cast<T*>(Pb[h]);
+    A.pslice = L.alloc;
     for (int a = 0; a < 3; ++a)
         A.pml[a] = pml[h][a];
     A.nunits = int(units[h].size());
-    for (int u = 0; u < A.nunits; ++u)
-        A.units[u] = units[h][u];
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) {
+            A.fbox[c][a][0] = 1 << 30;
+            A.fbox[c][a][1] = -(1 << 30);
+        }
+    for (int u = 0; u < A.nunits; ++u) {
+        const DftDev& d = units[h][u];
+        A.units[u] = d;
+        if (!d.fused || d.npts == 0)
+            continue;
+        const int c = d.comp;
+        A.fu[c][A.nfu[c]++] = u;
+        for (int a = 0; a < 3; ++a) {
+            A.fbox[c][a][0] = std::min(A.fbox[c][a][0], d.box[a][0]);
+            A.fbox[c][a][1] = std::max(A.fbox[c][a][1], d.box[a][1]);
+        }
+    }
     A.ph = ph[h] + size_t(n - ph_start) * 2 * std::max(1, MF[h]);
     // tile grid, padded to a multiple of the 8 XCDs
     A.lz = lz;
@@ -1151,19 +1389,13 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
     dim3 grid(A.nblocks, 1, 1);
     const bool mat = !S->mat[h].uniform;
     const int U = mat ? unroll_mat : unroll;
+    const size_t lds = mat ? size_t(A.nmats + A.npoles) * 3 * sizeof(double) : 0;
     if (e0)
         HIP_OK(hipEventRecord(e0, stream));
-    if (h == 0) {
-        if (mat)
-            launch_k<T, true, true>(U, grid, block, A);
-        else
-            launch_k<T, true, false>(U, grid, block, A);
-    } else {
-        if (mat)
-            launch_k<T, false, true>(U, grid, block, A);
-        else
-            launch_k<T, false, false>(U, grid, block, A);
-    }
+    if (h == 0)
+        launch_k<T, true>(mat, S_slots[h], U, lds, grid, block, A);
+    else
+        launch_k<T, false>(mat, S_slots[h], U, lds, grid, block, A);
     HIP_OK(hipGetLastError());
     if (e1)
         HIP_OK(hipEventRecord(e1, stream));
