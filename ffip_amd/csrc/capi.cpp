@@ -312,6 +312,17 @@ std::string plan_json(const Setup& S)
     JsonOut o;
     const Layout& L = S.L;
     o.s += "{";
+    // polarization storage the engine would use for each half
+    o.key("polarization");
+    o.s += "[";
+    for (int h = 0; h < 2; ++h) {
+        PolLayoutPlan P = plan_polarization(S, h);
+        char b[160];
+        std::snprintf(b, sizeof b, "%s{\"compact\":%d,\"full_bytes\":%.17g,\"compact_bytes\":%.17g}", h ? "," : "",
+                      P.compact ? 1 : 0, P.full_bytes, P.compact_bytes);
+        o.s += b;
+    }
+    o.s += "],";
     o.key("P1");
     o.s += "[" + std::to_string(L.P1[0]) + "," + std::to_string(L.P1[1]) + "," + std::to_string(L.P1[2]) + "],";
     o.key("P2");

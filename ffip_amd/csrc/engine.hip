@@ -108,11 +108,19 @@ struct HalfArgs {
     const double* d_eps[3];
     const double* d_amp[3];
     const double* d_c[3];  // [np][3] = (c1, c2, c3)
-    // polarization state of slot s, component c: slice s*3 + c of pcur (read) and of
-    // pprev (read, then overwritten with P^{n+1}); the buffers swap roles every step
+    // Polarization state.  pcur is read, pprev read and then overwritten with
+    // P^{n+1}; the buffers swap roles every step.  Two layouts:
+    //  full grid: pole s of component c at cell o is element (s*3 + c)*pslice + o;
+    //  compact:   only points with poles hold entries; component c's start at
+    //             pcomp[c], the row segment (k, j, x tile) holds its points' poles
+    //             point-major (in x order) from segbase[c][((k - dlo_z) * seg_ny +
+    //             (j - dlo_y)) * gx + tx].
     T* pcur;
     T* pprev;
-    int64_t pslice;  // elements per slice
+    int64_t pslice;
+    int64_t pcomp[3];
+    const uint32_t* segbase[3];
+    int seg_ny;
     int origin;  // cell offset of lattice index (0, 0, 0) in every per-cell array
     PmlDev pml[3];
     int nunits;
@@ -163,15 +171,33 @@ __device__ __forceinline__ void stb(T* base, uint32_t off, T v)
 // Everything one thread reads for one plane.  All of it is requested before any
 // of it is used, so a plane costs one memory round trip whatever it holds (CPML
 // auxiliaries, several poles, dense-region coefficients).
-template <typename T, int NPM>
+template <typename T, int NPM, bool CP>
 struct PlaneIn {
     T n0, n1, v2, v2a, v2b, v1a, v0b;  // stencil operands (see k_half)
     T e[3];                            // the three updated components
     T psi[3][2];                       // CPML auxiliaries of the two curl terms of each component
     T pc[3][NPM > 0 ? NPM : 1];        // P^n of each pole
     T pp[3][NPM > 0 ? NPM : 1];        // P^{n-1}
+    uint32_t poff[3][CP && NPM > 0 ? NPM : 1];  // compact P storage: entry of each pole
     double dinv[3];                    // 1/eps_inf in the dense region
 };
+
+// Rank of this lane among the lanes of its x-row segment whose predicate holds,
+// and their count (a row segment is BX consecutive lanes).
+__device__ __forceinline__ void row_rank(bool pred, uint32_t& rank, uint32_t& count)
+{
+    const uint64_t m = __ballot(pred);
+    const uint32_t lane = __lane_id();
+    static_assert(BX == 32 || BX == 64, "row segments are half or whole wavefronts");
+    if constexpr (BX == 32) {
+        const uint32_t row = (lane & 32u) ? uint32_t(m >> 32) : uint32_t(m);
+        rank = __popc(row & ((1u << (lane & 31u)) - 1u));
+        count = __popc(row);
+    } else {
+        rank = __popcll(m & ((uint64_t(1) << lane) - 1));
+        count = __popcll(m);
+    }
+}
 
 // CPML auxiliary offsets (slab-only arrays, src/fields.cpp:89-109): the slab along
 // axis a stores [W][n_b][n_c] with the slab position replacing that axis.
@@ -201,8 +227,8 @@ __device__ __forceinline__ int64_t dense_off(const HalfArgs<T>& A, int C, int i,
 
 // Stencil operands and updated components of one plane (loads every thread
 // issues; o is clamped into the segment so they are always in bounds).
-template <typename T, bool IS_E, int NPM>
-__device__ __forceinline__ void load_fields(const HalfArgs<T>& A, int o, PlaneIn<T, NPM>& L)
+template <typename T, bool IS_E, int NPM, bool CP>
+__device__ __forceinline__ void load_fields(const HalfArgs<T>& A, int o, PlaneIn<T, NPM, CP>& L)
 {
     constexpr uint32_t E = sizeof(T);
     const uint32_t b = uint32_t(o) * E, by = uint32_t(A.pitch) * E, bz = uint32_t(A.plane) * E;
@@ -231,12 +257,14 @@ __device__ __forceinline__ void load_fields(const HalfArgs<T>& A, int o, PlaneIn
 // 1/eps_inf.  A batch issues all of these before its unconditional loads, so
 // that waiting for the (newer) field data also covers them and the wait counts
 // stay exact whichever of these branches ran.  `id` is the packed material
-// word of the cell, fetched one batch earlier.
-template <typename T, bool MAT, int NPM>
+// word of the cell and (compact layout) sb[c] the first P entry of its row
+// segment, both fetched one batch earlier.  With the compact layout it must be
+// called by every live lane of the wave (offsets come from ballots over the row).
+template <typename T, bool MAT, int NPM, bool CP>
 __device__ __forceinline__ void load_sparse(const HalfArgs<T>& A, const MatEntry* smat, const ColCtx& X,
-                                            const ZCtx& Z, uint32_t id, const bool act[3], PlaneIn<T, NPM>& L)
+                                            const ZCtx& Z, uint32_t id, const uint32_t sb[3], const bool act[3],
+                                            PlaneIn<T, NPM, CP>& L)
 {
-    const int o = Z.o;
     // CPML auxiliaries: component 0 differentiates along (y, z), 1 along (z, x), 2 along (x, y)
     if (act[0]) {
         if (X.sy >= 0)
@@ -259,25 +287,52 @@ __device__ __forceinline__ void load_sparse(const HalfArgs<T>& A, const MatEntry
     if (MAT) {
 #pragma unroll
         for (int C = 0; C < 3; ++C) {
-            if (!act[C])
-                continue;
             const int mid = int((id >> (8 * C)) & 255u);
-            int np;
-            if (mid != 255) {
-                np = smat[mid].np;
-            } else {
-                np = A.d_np[C];
-                L.dinv[C] = A.d_inv[C][dense_off(A, C, X.i, X.j, Z.k)];
+            int np = 0;
+            if (act[C]) {
+                if (mid != 255) {
+                    np = smat[mid].np;
+                } else {
+                    np = A.d_np[C];
+                    L.dinv[C] = A.d_inv[C][dense_off(A, C, X.i, X.j, Z.k)];
+                }
             }
+            uint32_t e = CP ? sb[C] : 0u;
 #pragma unroll
             for (int s = 0; s < NPM; ++s) {
+                if constexpr (CP) {
+                    // pole s of the row's points in x order: this lane's rank among
+                    // those having it, after the entries of the lower poles
+                    uint32_t rank, count;
+                    row_rank(s < np, rank, count);
+                    L.poff[C][s] = e + rank;
+                    e += count;
+                }
                 if (s < np) {
-                    L.pc[C][s] = ldb<T>(A.pcur + (s * 3 + C) * A.pslice, uint32_t(o) * sizeof(T));
-                    L.pp[C][s] = ldb<T>(A.pprev + (s * 3 + C) * A.pslice, uint32_t(o) * sizeof(T));
+                    if constexpr (CP) {
+                        const uint32_t b = L.poff[C][s] * uint32_t(sizeof(T));
+                        L.pc[C][s] = ldb<T>(A.pcur + A.pcomp[C], b);
+                        L.pp[C][s] = ldb<T>(A.pprev + A.pcomp[C], b);
+                    } else {
+                        L.pc[C][s] = ldb<T>(A.pcur + (s * 3 + C) * A.pslice, uint32_t(Z.o) * sizeof(T));
+                        L.pp[C][s] = ldb<T>(A.pprev + (s * 3 + C) * A.pslice, uint32_t(Z.o) * sizeof(T));
+                    }
                 }
             }
         }
     }
+}
+
+// element (of the polarization buffers) holding pole s of component C at this point
+template <typename T, int NPM, bool CP>
+__device__ __forceinline__ T* pole_ptr(const HalfArgs<T>& A, const PlaneIn<T, NPM, CP>& L, const ZCtx& Z, int C, int s)
+{
+    if constexpr (CP)
+        return reinterpret_cast<T*>(reinterpret_cast<char*>(A.pprev + A.pcomp[C]) +
+                                    L.poff[C][s] * uint32_t(sizeof(T)));
+    else
+        return reinterpret_cast<T*>(reinterpret_cast<char*>(A.pprev + (s * 3 + C) * A.pslice) +
+                                    uint32_t(Z.o) * uint32_t(sizeof(T)));
 }
 
 // one CPML term (src/fields.cpp:89-109): psi <- b psi + c d; returns psi + d / kappa
@@ -291,10 +346,10 @@ __device__ __forceinline__ double cpml_apply(void* arr, int q, T psi, double b, 
 
 // Update component C of one point from its preloaded plane data: CPML, ADE
 // poles (src/structure.cpp:56-85, in E form), DFT accumulation, store.
-template <typename T, bool MAT, int NPM, int C>
+template <typename T, bool MAT, int NPM, bool CP, int C>
 __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry* smat, const double* spc,
                                             const ColCtx& X, const ZCtx& Z, double d1, double d2, uint32_t id,
-                                            const PlaneIn<T, NPM>& L)
+                                            const PlaneIn<T, NPM, CP>& L)
 {
     // term 1 differentiates along a1 = (C+1)%3, term 2 along a2 = (C+2)%3
     double t1 = d1, t2 = d2;
@@ -333,7 +388,7 @@ __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry
                     const double pc = double(L.pc[C][s]);
                     const double pn = cf[0] * pc + cf[1] * double(L.pp[C][s]) + cf[2] * e_old;
                     const T pt = T(pn);
-                    stb<T>(A.pprev + (s * 3 + C) * A.pslice, uint32_t(Z.o) * sizeof(T), pt);
+                    *pole_ptr<T, NPM, CP>(A, L, Z, C, s) = pt;
                     dP += double(pt) - pc;
                     sumP += double(pt);
                 }
@@ -351,7 +406,7 @@ __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry
                         const double pc = double(L.pc[C][s]);
                         const double pn = cc[0] * pc + cc[1] * double(L.pp[C][s]) + amp * cc[2] * e_old;
                         const T pt = T(pn);
-                        stb<T>(A.pprev + (s * 3 + C) * A.pslice, uint32_t(Z.o) * sizeof(T), pt);
+                        *pole_ptr<T, NPM, CP>(A, L, Z, C, s) = pt;
                         dP += double(pt) - pc;
                         sumP += double(pt);
                     }
@@ -393,7 +448,7 @@ __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry
 // Loads for U consecutive planes are issued before any of them is computed so
 // that each thread keeps several planes of requests in flight; the two
 // components differentiated along z are carried between planes in registers.
-template <typename T, bool IS_E, bool MAT, int NPM, int U>
+template <typename T, bool IS_E, bool MAT, int NPM, bool CP, int U>
 __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
 {
     // material table and pole coefficients, staged in LDS (sized at launch:
@@ -457,16 +512,29 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
         c1 = ldb(A.s[1], b);
     }
     // material words are fetched one batch ahead of the planes they describe
-    uint32_t idn[U];
+    // and so are the first compact polarization entries of the row segments
+    constexpr bool POL = MAT && NPM > 0 && CP;
+    const int seg0 = (j - A.dlo[1]) * A.gx + tx;
+    const int segplane = A.seg_ny * A.gx;
+    uint32_t idn[U], sbn[U][3];
 #pragma unroll
-    for (int q = 0; q < U; ++q)
-        idn[q] = MAT ? ldb(A.ids, uint32_t(X.base + sz * min(kb + q, ke)) * 4u) : 0u;
+    for (int q = 0; q < U; ++q) {
+        const int kk = min(kb + q, ke);
+        idn[q] = MAT ? ldb(A.ids, uint32_t(X.base + sz * kk) * 4u) : 0u;
+#pragma unroll
+        for (int C = 0; C < 3; ++C)
+            sbn[q][C] = POL ? A.segbase[C][(kk - A.dlo[2]) * segplane + seg0] : 0u;
+    }
     for (int kc = kb; kc <= ke; kc += U) {
-        uint32_t idc[U];
+        uint32_t idc[U], sbc[U][3];
 #pragma unroll
-        for (int q = 0; q < U; ++q)
+        for (int q = 0; q < U; ++q) {
             idc[q] = idn[q];
-        PlaneIn<T, NPM> L[U];
+#pragma unroll
+            for (int C = 0; C < 3; ++C)
+                sbc[q][C] = sbn[q][C];
+        }
+        PlaneIn<T, NPM, CP> L[U];
         ZCtx Z[U];
         bool act[U][3];
 #pragma unroll
@@ -489,37 +557,43 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
             for (int C = 0; C < 3; ++C)
                 act[q][C] = k <= ke && X.in[C] && k >= A.lo[C][2] && k <= A.hi[C][2];
             if (k <= ke)
-                load_sparse<T, MAT, NPM>(A, smat, X, Z[q], idc[q], act[q], L[q]);
+                load_sparse<T, MAT, NPM, CP>(A, smat, X, Z[q], idc[q], sbc[q], act[q], L[q]);
         }
-        // then the loads every thread makes: next batch's material words, fields
+        // then the loads every thread makes: next batch's material words and
+        // segment bases, fields
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int kk = min(kc + U + q, ke);
+            idn[q] = MAT ? ldb(A.ids, uint32_t(X.base + sz * kk) * 4u) : 0u;
+#pragma unroll
+            for (int C = 0; C < 3; ++C)
+                sbn[q][C] = POL ? A.segbase[C][(kk - A.dlo[2]) * segplane + seg0] : 0u;
+        }
 #pragma unroll
         for (int q = 0; q < U; ++q)
-            idn[q] = MAT ? ldb(A.ids, uint32_t(X.base + sz * min(kc + U + q, ke)) * 4u) : 0u;
-#pragma unroll
-        for (int q = 0; q < U; ++q)
-            load_fields<T, IS_E, NPM>(A, X.base + sz * min(kc + q, ke), L[q]);
+            load_fields<T, IS_E, NPM, CP>(A, X.base + sz * min(kc + q, ke), L[q]);
 #pragma unroll
         for (int q = 0; q < U; ++q) {
             if (kc + q > ke)
                 break;
-            const PlaneIn<T, NPM>& P = L[q];
+            const PlaneIn<T, NPM, CP>& P = L[q];
             // E: s0(k)=c0, s0(k+1)=n0; H: s0(k-1)=c0, s0(k)=n0 (s1 likewise)
             const double a0 = double(c0), a1 = double(c1), g0 = double(P.n0), g1 = double(P.n1);
             const double w2 = double(P.v2);
             if (act[q][0]) {
                 const double d1 = (double(P.v2b) - w2) * inv_dx;
                 const double d2 = (IS_E ? (g1 - a1) : (a1 - g1)) * inv_dx;
-                update_comp<T, MAT, NPM, 0>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
+                update_comp<T, MAT, NPM, CP, 0>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
             if (act[q][1]) {
                 const double d1 = (IS_E ? (g0 - a0) : (a0 - g0)) * inv_dx;
                 const double d2 = (double(P.v2a) - w2) * inv_dx;
-                update_comp<T, MAT, NPM, 1>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
+                update_comp<T, MAT, NPM, CP, 1>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
             if (act[q][2]) {
                 const double d1 = (double(P.v1a) - (IS_E ? a1 : g1)) * inv_dx;
                 const double d2 = (double(P.v0b) - (IS_E ? a0 : g0)) * inv_dx;
-                update_comp<T, MAT, NPM, 2>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
+                update_comp<T, MAT, NPM, CP, 2>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
             c0 = P.n0;
             c1 = P.n1;
@@ -839,6 +913,9 @@ struct EngineImpl {
     void* Pa[2] = {nullptr, nullptr};
     void* Pb[2] = {nullptr, nullptr};
     int S_slots[2] = {0, 0};
+    int64_t pcomp[2][3] = {};           // element offset of each component's compact P entries
+    uint32_t* segbase[2][3] = {};       // first entry of every BX-point row segment
+    int pseg_ny[2] = {0, 0};
     uint32_t* ids[2] = {};  // packed ids of the three components of a half
     MatEntry* mats[2] = {};
     int nmats[2] = {0, 0};
@@ -923,37 +1000,49 @@ struct EngineImpl {
     template <typename T>
     void launch_half(int h, hipEvent_t e0, hipEvent_t e1);
     // npm: most poles at any point of the half (register arrays are sized by it);
-    // U: planes whose loads are issued together
-    template <typename T, bool IS_E>
-    void launch_k(bool mat, int npm, int U, size_t lds, dim3 grid, dim3 block, const HalfArgs<T>& A)
+    // cp: compact polarization layout; U: planes whose loads are issued together
+    template <typename T, bool IS_E, bool CP>
+    void launch_mat(int npm, int U, size_t lds, dim3 grid, dim3 block, const HalfArgs<T>& A)
     {
-        if (!mat) {
-            switch (U) {
-            case 1: k_half<T, IS_E, false, 0, 1><<<grid, block, 0, stream>>>(A); break;
-            case 2: k_half<T, IS_E, false, 0, 2><<<grid, block, 0, stream>>>(A); break;
-            default: k_half<T, IS_E, false, 0, 4><<<grid, block, 0, stream>>>(A); break;
-            }
-            return;
-        }
-        if (U >= 2 && npm <= 4) {
-            switch (npm) {
-            case 0: k_half<T, IS_E, true, 0, 2><<<grid, block, lds, stream>>>(A); break;
-            case 1: k_half<T, IS_E, true, 1, 2><<<grid, block, lds, stream>>>(A); break;
-            case 2: k_half<T, IS_E, true, 2, 2><<<grid, block, lds, stream>>>(A); break;
-            case 3: k_half<T, IS_E, true, 3, 2><<<grid, block, lds, stream>>>(A); break;
-            default: k_half<T, IS_E, true, 4, 2><<<grid, block, lds, stream>>>(A); break;
-            }
+#define FFIP_K(NP, UU) k_half<T, IS_E, true, NP, CP, UU><<<grid, block, lds, stream>>>(A)
+        // two planes per batch only while the pole registers leave occupancy 3
+        if (U >= 2 && npm <= 2) {
+            if (npm == 1)
+                FFIP_K(1, 2);
+            else
+                FFIP_K(2, 2);
             return;
         }
         switch (npm) {
-        case 0: k_half<T, IS_E, true, 0, 1><<<grid, block, lds, stream>>>(A); break;
-        case 1: k_half<T, IS_E, true, 1, 1><<<grid, block, lds, stream>>>(A); break;
-        case 2: k_half<T, IS_E, true, 2, 1><<<grid, block, lds, stream>>>(A); break;
-        case 3: k_half<T, IS_E, true, 3, 1><<<grid, block, lds, stream>>>(A); break;
-        case 4: k_half<T, IS_E, true, 4, 1><<<grid, block, lds, stream>>>(A); break;
-        default: k_half<T, IS_E, true, MAXS, 1><<<grid, block, lds, stream>>>(A); break;
+        case 1: FFIP_K(1, 1); break;
+        case 2: FFIP_K(2, 1); break;
+        case 3: FFIP_K(3, 1); break;
+        case 4: FFIP_K(4, 1); break;
+        default: FFIP_K(MAXS, 1); break;
+        }
+#undef FFIP_K
+    }
+    template <typename T, bool IS_E>
+    void launch_k(bool mat, int npm, bool cp, int U, size_t lds, dim3 grid, dim3 block, const HalfArgs<T>& A)
+    {
+        if (!mat) {
+            switch (U) {
+            case 1: k_half<T, IS_E, false, 0, false, 1><<<grid, block, 0, stream>>>(A); break;
+            case 2: k_half<T, IS_E, false, 0, false, 2><<<grid, block, 0, stream>>>(A); break;
+            default: k_half<T, IS_E, false, 0, false, 4><<<grid, block, 0, stream>>>(A); break;
+            }
+        } else if (npm == 0) {
+            if (U >= 2)
+                k_half<T, IS_E, true, 0, false, 2><<<grid, block, lds, stream>>>(A);
+            else
+                k_half<T, IS_E, true, 0, false, 1><<<grid, block, lds, stream>>>(A);
+        } else if (cp) {
+            launch_mat<T, IS_E, true>(npm, U, lds, grid, block, A);
+        } else {
+            launch_mat<T, IS_E, false>(npm, U, lds, grid, block, A);
         }
     }
+    bool pcompact[2] = {false, false};  // polarization layout of each half
     int unroll = 4, unroll_mat = 1;  // z planes whose loads are issued together
     void ensure_phasors();
     template <typename T>
@@ -970,6 +1059,62 @@ EngineImpl::~EngineImpl()
         hipFree(p);
     if (stream)
         hipStreamDestroy(stream);
+}
+
+PolLayoutPlan plan_polarization(const Setup& s, int h)
+{
+    const Layout& L = s.L;
+    const HalfMaterial& M = s.mat[h];
+    const size_t tsz = s.precision == "fp32" ? 4 : 8;
+    PolLayoutPlan P;
+    if (M.uniform || M.max_poles == 0)
+        return P;
+    for (int a = 0; a < 3; ++a) {
+        P.dlo[a] = 1 << 30;
+        P.dhi[a] = -1;
+        for (int c = 0; c < 3; ++c)
+            if (L.hi[h * 3 + c][a] >= L.lo[h * 3 + c][a]) {
+                P.dlo[a] = std::min(P.dlo[a], L.lo[h * 3 + c][a]);
+                P.dhi[a] = std::max(P.dhi[a], L.hi[h * 3 + c][a]);
+            }
+    }
+    P.gx = std::max(0, (P.dhi[0] - P.dlo[0] + BX) / BX);
+    P.ny = std::max(0, P.dhi[1] - P.dlo[1] + 1);
+    P.nz = std::max(0, P.dhi[2] - P.dlo[2] + 1);
+    const size_t nseg = size_t(P.gx) * P.ny * P.nz;
+    for (int c = 0; c < 3; ++c) {
+        const int comp = h * 3 + c;
+        std::vector<uint32_t>& sb = P.segbase[c];
+        sb.assign(std::max<size_t>(nseg, 1), 0u);
+        int64_t run = 0;
+        size_t seg = 0;
+        for (int k = P.dlo[2]; k <= P.dhi[2]; ++k)
+            for (int j = P.dlo[1]; j <= P.dhi[1]; ++j)
+                for (int tx = 0; tx < P.gx; ++tx, ++seg) {
+                    sb[seg] = uint32_t(run);
+                    if (j < L.lo[comp][1] || j > L.hi[comp][1] || k < L.lo[comp][2] || k > L.hi[comp][2])
+                        continue;
+                    const int i0 = std::max(P.dlo[0] + tx * BX, L.lo[comp][0]);
+                    const int i1 = std::min(P.dlo[0] + tx * BX + BX - 1, L.hi[comp][0]);
+                    int most = 0, n = 0;
+                    for (int i = i0; i <= i1; ++i) {
+                        const int id = M.ids[c][L.index(i, j, k)];
+                        const int np = id == 255 ? M.dense[c].npoles : M.table.npoles[id];
+                        n += np;
+                        most = std::max(most, np);
+                    }
+                    run += n;
+                    P.full_bytes += double(most) * BX * double(tsz);
+                    // a segment's entries are contiguous: whole lines but a partial one
+                    P.compact_bytes += n ? double(n) * double(tsz) + 64.0 : 0.0;
+                }
+        if (run * int64_t(tsz) >= (int64_t(1) << 32))
+            throw std::runtime_error("polarization state of one component exceeds 4 GiB");
+        P.comp_start[c] = P.total;
+        P.total += run;
+    }
+    P.compact = P.full_bytes > 1.5 * P.compact_bytes;
+    return P;
 }
 
 void EngineImpl::build()
@@ -1005,7 +1150,19 @@ void EngineImpl::build()
         if (S_slots[h] > MAXS)
             throw std::runtime_error("more than 8 polarization poles at one point");
         if (S_slots[h] > 0) {
-            size_t cnt = size_t(S_slots[h]) * 3 * size_t(L.alloc);
+            PolLayoutPlan P = plan_polarization(s, h);
+            pcompact[h] = P.compact;
+            size_t cnt;
+            if (P.compact) {
+                for (int c = 0; c < 3; ++c) {
+                    pcomp[h][c] = P.comp_start[c];
+                    segbase[h][c] = upload(P.segbase[c]);
+                }
+                pseg_ny[h] = P.ny;
+                cnt = size_t(P.total);
+            } else {
+                cnt = size_t(S_slots[h]) * 3 * size_t(L.alloc);
+            }
             Pa[h] = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
             Pb[h] = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
         }
@@ -1331,6 +1488,11 @@ HalfArgs<T> EngineImpl::half_args(int h)
     A.pcur = static_cast<T*>(Pa[h]);
     A.pprev = static_cast<T*>(Pb[h]);
     A.pslice = L.alloc;
+    for (int c = 0; c < 3; ++c) {
+        A.pcomp[c] = pcomp[h][c];
+        A.segbase[c] = segbase[h][c];
+    }
+    A.seg_ny = pseg_ny[h];
     for (int a = 0; a < 3; ++a)
         A.pml[a] = pml[h][a];
     A.nunits = int(units[h].size());
@@ -1393,9 +1555,9 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
     if (e0)
         HIP_OK(hipEventRecord(e0, stream));
     if (h == 0)
-        launch_k<T, true>(mat, S_slots[h], U, lds, grid, block, A);
+        launch_k<T, true>(mat, S_slots[h], pcompact[h], U, lds, grid, block, A);
     else
-        launch_k<T, false>(mat, S_slots[h], U, lds, grid, block, A);
+        launch_k<T, false>(mat, S_slots[h], pcompact[h], U, lds, grid, block, A);
     HIP_OK(hipGetLastError());
     if (e1)
         HIP_OK(hipEventRecord(e1, stream));

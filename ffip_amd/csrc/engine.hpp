@@ -13,6 +13,23 @@ namespace ffip_amd {
 
 struct EngineImpl;
 
+// Polarization storage of one half.  The half-step tiles cut every x-row into
+// segments of a fixed number of points.  Full grid: one array per (pole,
+// component); a segment fetches a pole's whole span as soon as one of its points
+// has that pole.  Compact: only points with poles hold entries, each segment's
+// slot-major (pole 0 of its points in x order, then pole 1, ...).  Compact is
+// chosen when the full grid would move more than 1.5x its bytes.
+struct PolLayoutPlan {
+    bool compact = false;
+    double full_bytes = 0, compact_bytes = 0;  // estimated P bytes per sweep (one read)
+    int dlo[3] = {0, 0, 0}, dhi[3] = {-1, -1, -1};
+    int gx = 0, ny = 0, nz = 0;               // segments per row, rows, planes
+    int64_t comp_start[3] = {0, 0, 0};        // compact: first entry of each component
+    int64_t total = 0;                        // compact: entries of the half
+    std::vector<uint32_t> segbase[3];         // compact: first entry of every segment
+};
+PolLayoutPlan plan_polarization(const Setup& S, int half);
+
 struct KernelTiming {
     double total_ms = 0;  // wall time of the timed region (HIP events)
     double e_ms = 0;      // summed duration of the electric half-step kernel

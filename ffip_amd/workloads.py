@@ -178,3 +178,54 @@ def design_box(grid=(200, 200, 100), pml_cells=10, design=(100, 100, 20), patch=
     cfg["fields output"] = [dft((0, 0, patch_z), (patch[0], patch[1], 0), freqs, "Ez", "volume fields dft 0")]
     cfg["stop condition"] = {"type": "time", "time": steps * 0.5}
     return cfg, {"two medium box adjoint": rho.ravel()}
+
+
+def voxel_mix(n=512, voxel=8, seed=5, pml_cells=16, steps=10000, precision="fp32", nfreq=0, nplanes=8,
+              lorentz_frac=0.3, drude_frac=0.1, source_offset=24):
+    """Large single-GPU workload: n^3 cells, CPML, a Gaussian1 Ez current sheet at
+    z = -n/2 + source_offset, and a seeded mix of voxel-sized boxes in vacuum:
+    `lorentz_frac` of the voxel^3-cell voxels hold a 3-pole Lorentz dielectric
+    drawn from a palette of 8 (eps_inf = 2, f0 in U[0.05,0.2], gamma in
+    U[0.005,0.02], delta-eps in U[0.5,2]), `drude_frac` a Drude metal from a
+    palette of 4 (omega_p in U[0.8,1.2] rad per unit time, gamma = 0.01).  Voxels
+    are box geometries a quarter cell larger than the voxel on every side, so the
+    points on shared faces belong to one of them (first in list order) instead of
+    to neither.  With nfreq > 0, `nplanes` Ez DFT planes at half-integer z carry
+    nfreq frequencies in [0.05, 0.15].  28 distinct poles: within the
+    reference's 31-pole pool."""
+    rng = np.random.default_rng(seed)
+    f0 = rng.uniform(0.05, 0.2, size=(8, 3))
+    g0 = rng.uniform(0.005, 0.02, size=(8, 3))
+    a0 = rng.uniform(0.5, 2.0, size=(8, 3))
+    wp = rng.uniform(0.8, 1.2, size=4)
+    lor = [medium(2.0, e_sus=[lorentz(f0[q, p], g0[q, p], a0[q, p]) for p in range(3)]) for q in range(8)]
+    dru = [medium(1.0, e_sus=[drude(wp[q] / (2 * math.pi), 0.01, 1.0)]) for q in range(4)]
+    nv = n // voxel
+    u = rng.random(nv ** 3)
+    pick = rng.integers(0, 8, size=nv ** 3)
+    pick_d = rng.integers(0, 4, size=nv ** 3)
+    geoms = []
+    half = voxel / 2 + 0.25
+    for q in range(nv ** 3):
+        if u[q] < lorentz_frac:
+            m = lor[pick[q]]
+        elif u[q] < lorentz_frac + drude_frac:
+            m = dru[pick_d[q]]
+        else:
+            continue
+        iz, r = divmod(q, nv * nv)
+        iy, ix = divmod(r, nv)
+        c = [(ix + 0.5) * voxel - n / 2, (iy + 0.5) * voxel - n / 2, (iz + 0.5) * voxel - n / 2]
+        geoms.append({"type": "box", "center": c, "size": [2 * half] * 3, "medium": m})
+    cfg = base((n, n, n), pml_cells, precision)
+    cfg["geometry"] = geoms
+    cfg["sources"] = [{"type": "volume source", "center": [0.0, 0.0, -n / 2 + source_offset],
+                       "size": [float(n), float(n), 0.0], "amplitude": 1.0, "field component": "Ez",
+                       "function": gaussian1(0.1)}]
+    if nfreq > 0:
+        freqs = list(np.linspace(0.05, 0.15, nfreq))
+        zs = np.linspace(-n / 4, n / 4, nplanes)
+        cfg["fields output"] = [dft((0, 0, float(np.floor(z)) + 0.5), (n, n, 0), freqs, "Ez", "volume fields dft %d" % q)
+                                for q, z in enumerate(zs)]
+    cfg["stop condition"] = {"type": "time", "time": steps * 0.5}
+    return cfg

@@ -70,6 +70,10 @@ def _cases():
     cfg["sources"] = [{"type": "volume source", "center": [2.0, 0.0, 0.0], "size": [0.0, 0.0, 0.0],
                        "amplitude": 1.0, "field component": "Ey", "function": W.gaussian1(0.1)}]
     c["symmetry"] = (cfg, None, [])
+    # the large-grid recipe at a size the oracle runs: ~200 overlapping voxel
+    # boxes, 3-pole Lorentz and Drude palettes, z-normal DFT planes
+    c["voxels"] = (W.voxel_mix(n=32, voxel=4, pml_cells=5, nfreq=2, nplanes=2, precision="fp64", source_offset=7),
+                   None, [((1.5, -2.0, 3.0), "Ez")])
     return c
 
 
