@@ -57,6 +57,23 @@ _SIGS = {
     "ffip_transpose": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                         ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
     "ffip_sim_destroy": ([ctypes.c_void_p], None),
+    # surrogate kernels (device pointers, stream as void*)
+    "ffip_sg_gemm": ([ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                      ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                      ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sg_im2col": ([ctypes.c_void_p] * 4, ctypes.c_int),
+    "ffip_sg_col2im": ([ctypes.c_void_p] * 4, ctypes.c_int),
+    "ffip_sg_bias_act": ([ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sg_act_back": ([ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                          ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sg_colsum": ([ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                        ctypes.c_void_p], ctypes.c_int),
+    "ffip_sg_to_bf16": ([ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sg_head": ([ctypes.c_void_p] * 4, ctypes.c_int),
+    "ffip_sg_head_back": ([ctypes.c_void_p] * 4, ctypes.c_int),
+    "ffip_sg_adam": ([ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                      ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
 }
 
 

@@ -11,6 +11,7 @@
 #include "engine.hpp"
 #include "json.hpp"
 #include "setup.hpp"
+#include "surrogate.hpp"
 
 using namespace ffip_amd;
 
@@ -596,6 +597,84 @@ int ffip_transpose(const double* gx, int64_t nx, const double* gy, int64_t ny, c
             transpose_axis(out, dims, strides, q, 3, p[3]);
         }
     });
+}
+
+// ------------------------------------------------------------ surrogate kernels
+int ffip_sg_gemm(int ta, int tb, int64_t M, int64_t N, int64_t K, const void* A, int64_t lda, const void* B,
+                 int64_t ldb, float* C, int64_t ldc, int accumulate, int ksplit, void* stream)
+{
+    if ((M > 0 && N > 0 && (!C || (K > 0 && (!A || !B)))) || lda < 0 || ldb < 0 || ldc < N)
+        return fail(FFIP_EINVAL, "bad GEMM arguments");
+    return guard([&] { sg_gemm(ta, tb, M, N, K, A, lda, B, ldb, C, ldc, accumulate, ksplit, stream); });
+}
+
+int ffip_sg_im2col(const int64_t geom[11], const void* X, void* col, void* stream)
+{
+    if (!geom || !X || !col)
+        return fail(FFIP_EINVAL, "null argument");
+    return guard([&] { sg_im2col(geom, X, col, stream); });
+}
+
+int ffip_sg_col2im(const int64_t geom[11], const float* dcol, float* dX, void* stream)
+{
+    if (!geom || !dcol || !dX)
+        return fail(FFIP_EINVAL, "null argument");
+    return guard([&] { sg_col2im(geom, dcol, dX, stream); });
+}
+
+int ffip_sg_bias_act(int64_t rows, int64_t n, const float* Y, const float* bias, int relu, void* out, float* zkeep,
+                     void* stream)
+{
+    if (!Y || !bias || !out || rows < 0 || n < 1)
+        return fail(FFIP_EINVAL, "bad arguments");
+    return guard([&] { sg_bias_act(rows, n, Y, bias, relu, out, zkeep, stream); });
+}
+
+int ffip_sg_act_back(int64_t total, const float* dA, const float* Z, int relu, void* dZb, float* dZf, void* stream)
+{
+    if (!dA || !Z || !dZb || !dZf || total < 0)
+        return fail(FFIP_EINVAL, "bad arguments");
+    return guard([&] { sg_act_back(total, dA, Z, relu, dZb, dZf, stream); });
+}
+
+int ffip_sg_colsum(int64_t rows, int64_t n, const float* dZ, float* db, int accumulate, void* stream)
+{
+    if (!dZ || !db || rows < 0 || n < 1)
+        return fail(FFIP_EINVAL, "bad arguments");
+    return guard([&] { sg_colsum(rows, n, dZ, db, accumulate, stream); });
+}
+
+int ffip_sg_to_bf16(int64_t n, const float* x, void* y, void* stream)
+{
+    if (!x || !y || n < 0)
+        return fail(FFIP_EINVAL, "bad arguments");
+    return guard([&] { sg_to_bf16(n, x, y, stream); });
+}
+
+int ffip_sg_head(const int64_t h[9], const void* A, float* out, void* stream)
+{
+    if (!h || !A || !out)
+        return fail(FFIP_EINVAL, "null argument");
+    if (h[5] < 0 || h[6] < 0 || h[5] + h[7] > h[2] || h[6] + h[8] > h[3])
+        return fail(FFIP_EINVAL, "head crop outside the activation");
+    return guard([&] { sg_head(h, A, out, stream); });
+}
+
+int ffip_sg_head_back(const int64_t h[9], const float* dout, float* dA, void* stream)
+{
+    if (!h || !dout || !dA)
+        return fail(FFIP_EINVAL, "null argument");
+    if (h[5] < 0 || h[6] < 0 || h[5] + h[7] > h[2] || h[6] + h[8] > h[3])
+        return fail(FFIP_EINVAL, "head crop outside the activation");
+    return guard([&] { sg_head_back(h, dout, dA, stream); });
+}
+
+int ffip_sg_adam(int64_t n, float* w, const float* g, float* m, float* v, const float hp[4], int64_t t, void* wb,
+                 void* stream)
+{
+    if (!w || !g || !m || !v || !hp || n < 0 || t < 1)
+        return fail(FFIP_EINVAL, "bad arguments");
+    return guard([&] { sg_adam(n, w, g, m, v, hp, t, wb, stream); });
 }
 
 }  // extern "C"

@@ -1,0 +1,329 @@
+"""Design surrogate on the matrix cores.
+
+No counterpart in the reference.  A small 3-D convolutional network maps the
+design density of the inverse-design workload (21 x 101 x 101 density nodes of a
+100 x 100 x 20-cell two-medium box) to the complex Ez DFT on the 21 x 21 objective
+patch at four frequencies.  It is trained on (density, patch DFT) pairs that the
+GPU FDTD solver produces, and it proposes step lengths and inner steps of an
+adjoint gradient ascent; every design it proposes is re-judged by the full FDTD
+solve (and the adjoint solve when accepted), which stays the fallback.
+
+Network: four 3x3x3 convolutions (channels 1 -> 32 -> 32 -> 32 -> 8), each
+striding 2 along z (21 -> 11 -> 6 -> 3 -> 2 planes) with x/y dilations 1, 2, 4, 8
+(receptive field +-15 cells), ReLU after the first three; the head averages over
+z and crops the 21 x 21 patch; the 8 output channels are (Re, Im) at 4
+frequencies.  Each convolution runs as im2col + a BF16 MFMA GEMM with FP32
+accumulation (ffip_sg_*), its backward pass as two more GEMMs + col2im; Adam
+updates FP32 master weights and refreshes their bf16 copies.  torch only holds
+the device memory and supplies the stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import FfipError, _check
+
+FREQS = (0.09, 0.095, 0.1, 0.105)
+PATCH = 21  # objective patch nodes per side (20 x 20 cells)
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise FfipError("the surrogate kernels need the GPU; there is no CPU path")
+    return torch
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _arr(vals):
+    a = (ctypes.c_int64 * len(vals))(*[int(v) for v in vals])
+    return ctypes.cast(a, ctypes.c_void_p), a  # keep the array alive with the pointer
+
+
+def _stream():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+# ---------------------------------------------------------------- kernel calls
+def gemm(A, B, C, M, N, K, ta=False, tb=False, lda=None, ldb=None, ldc=None, accumulate=False, ksplit=1):
+    """C (+)= op(A) op(B) on the matrix cores (A, B bf16 tensors, C FP32)."""
+    lda = lda if lda is not None else (M if ta else K)
+    ldb = ldb if ldb is not None else (K if tb else N)
+    ldc = ldc if ldc is not None else N
+    _check(_lib.lib().ffip_sg_gemm(int(ta), int(tb), M, N, K, _vp(A), lda, _vp(B), ldb, _vp(C), ldc,
+                                   int(accumulate), int(ksplit), _stream()))
+
+
+def conv_out(shape, stride):
+    return tuple((n - 1) // s + 1 for n, s in zip(shape, stride))
+
+
+class Conv3:
+    """One 3x3x3 convolution layer, channels-last, weights [27*cin, cout]."""
+
+    def __init__(self, cin, cout, stride, dil, relu, rng):
+        torch = _torch()
+        self.cin, self.cout, self.stride, self.dil, self.relu = cin, cout, tuple(stride), tuple(dil), relu
+        K = 27 * cin
+        w = rng.standard_normal((K, cout)) * math.sqrt(2.0 / K)
+        dev = "cuda"
+        self.w = torch.tensor(w, dtype=torch.float32, device=dev)
+        self.wb = self.w.to(torch.bfloat16)
+        self.b = torch.zeros(cout, dtype=torch.float32, device=dev)
+        self.gw = torch.zeros_like(self.w)
+        self.gb = torch.zeros_like(self.b)
+        self.m = [torch.zeros_like(self.w), torch.zeros_like(self.b)]
+        self.v = [torch.zeros_like(self.w), torch.zeros_like(self.b)]
+        self.cache = None
+
+    def geom(self, batch, shape):
+        return [batch, self.cin] + list(shape) + list(self.stride) + list(self.dil)
+
+    def forward(self, x, batch, shape, keep):
+        torch = _torch()
+        out = conv_out(shape, self.stride)
+        M = batch * out[0] * out[1] * out[2]
+        K = 27 * self.cin
+        col = torch.empty((M, K), dtype=torch.bfloat16, device="cuda")
+        g, keep_g = _arr(self.geom(batch, shape))
+        _check(_lib.lib().ffip_sg_im2col(g, _vp(x), _vp(col), _stream()))
+        Y = torch.empty((M, self.cout), dtype=torch.float32, device="cuda")
+        gemm(col, self.wb, Y, M, self.cout, K)
+        a = torch.empty((M, self.cout), dtype=torch.bfloat16, device="cuda")
+        Z = torch.empty_like(Y) if keep else None
+        _check(_lib.lib().ffip_sg_bias_act(M, self.cout, _vp(Y), _vp(self.b), int(self.relu), _vp(a),
+                                           _vp(Z) if Z is not None else None, _stream()))
+        self.cache = (batch, shape, col, Z, M) if keep else None
+        return a, out
+
+    def backward(self, dA, need_dx=True):
+        """dA: FP32 [M, cout] gradient of this layer's output; accumulates gw, gb;
+        returns the FP32 gradient of the input [batch, z, y, x, cin]."""
+        torch = _torch()
+        batch, shape, col, Z, M = self.cache
+        K = 27 * self.cin
+        dZb = torch.empty((M, self.cout), dtype=torch.bfloat16, device="cuda")
+        dZf = torch.empty((M, self.cout), dtype=torch.float32, device="cuda")
+        _check(_lib.lib().ffip_sg_act_back(M * self.cout, _vp(dA), _vp(Z), int(self.relu), _vp(dZb), _vp(dZf),
+                                           _stream()))
+        _check(_lib.lib().ffip_sg_colsum(M, self.cout, _vp(dZf), _vp(self.gb), 0, _stream()))
+        # weight gradient col^T dZ: long K (= positions), split over workgroups
+        gemm(col, dZb, self.gw, K, self.cout, M, ta=True, lda=K, ldb=self.cout, ksplit=max(1, M // 8192))
+        if not need_dx:
+            return None
+        dcol = torch.empty((M, K), dtype=torch.float32, device="cuda")
+        gemm(dZb, self.wb, dcol, M, K, self.cout, tb=True, lda=self.cout, ldb=self.cout)
+        dX = torch.empty((batch,) + tuple(shape) + (self.cin,), dtype=torch.float32, device="cuda")
+        g, keep_g = _arr(self.geom(batch, shape))
+        _check(_lib.lib().ffip_sg_col2im(g, _vp(dcol), _vp(dX), _stream()))
+        return dX
+
+    def adam(self, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+        hp = (ctypes.c_float * 4)(lr, b1, b2, eps)
+        hpp = ctypes.cast(hp, ctypes.c_void_p)
+        _check(_lib.lib().ffip_sg_adam(self.w.numel(), _vp(self.w), _vp(self.gw), _vp(self.m[0]), _vp(self.v[0]),
+                                       hpp, t, _vp(self.wb), _stream()))
+        _check(_lib.lib().ffip_sg_adam(self.b.numel(), _vp(self.b), _vp(self.gb), _vp(self.m[1]), _vp(self.v[1]),
+                                       hpp, t, None, _stream()))
+
+
+class Surrogate:
+    """density [B, 21, 101, 101] in [0, 1] -> patch fields [B, 21, 21, 8] (Re, Im x 4 freqs)."""
+
+    def __init__(self, seed=0, shape=(21, 101, 101), crop=(40, 40, PATCH, PATCH), width=32, nout=2 * len(FREQS)):
+        rng = np.random.default_rng(seed)
+        self.shape, self.crop, self.nout = tuple(shape), crop, nout
+        dil = [(1, 1, 1), (1, 2, 2), (1, 4, 4), (1, 8, 8)]
+        chans = [1, width, width, width, nout]
+        self.layers = [Conv3(chans[q], chans[q + 1], (2, 1, 1), dil[q], q < 3, rng) for q in range(4)]
+        self.scale = 1.0  # target normalisation (fields / scale are learned)
+        self.t = 0
+        self._head = None
+
+    def _input(self, rho):
+        torch = _torch()
+        r = torch.as_tensor(np.asarray(rho, dtype=np.float32), device="cuda")
+        return (2.0 * r - 1.0).to(torch.bfloat16).contiguous()  # [B, z, y, x] == channels-last with C = 1
+
+    def forward(self, rho, keep=False):
+        torch = _torch()
+        x = self._input(rho)
+        batch = x.shape[0]
+        shape = self.shape
+        for L in self.layers:
+            x, shape = L.forward(x, batch, shape, keep)
+        y0, x0, ny, nx = self.crop
+        h = [batch, shape[0], shape[1], shape[2], self.nout, y0, x0, ny, nx]
+        self._head = (h, shape, batch)
+        out = torch.empty((batch, ny, nx, self.nout), dtype=torch.float32, device="cuda")
+        hp, keep_h = _arr(h)
+        _check(_lib.lib().ffip_sg_head(hp, _vp(x), _vp(out), _stream()))
+        return out
+
+    def backward(self, dout, need_dx=False):
+        torch = _torch()
+        h, shape, batch = self._head
+        dA = torch.empty((batch,) + tuple(shape) + (self.nout,), dtype=torch.float32, device="cuda")
+        hp, keep_h = _arr(h)
+        _check(_lib.lib().ffip_sg_head_back(hp, _vp(dout.contiguous()), _vp(dA), _stream()))
+        g = dA.reshape(-1, self.nout)
+        for q in range(len(self.layers) - 1, -1, -1):
+            L = self.layers[q]
+            dx = L.backward(g, need_dx=need_dx or q > 0)
+            if q > 0:
+                g = dx.reshape(-1, L.cin)
+            elif need_dx:
+                return dx.reshape((batch,) + tuple(self.shape)) * 2.0  # d/d rho of the 2 rho - 1 input map
+        return None
+
+    # ----------------------------------------------------------- training
+    def fit(self, rhos, targets, epochs=30, batch=8, lr=1e-3, seed=0, log=None):
+        """Mean-squared error on targets [N, 21, 21, 8] (fields divided by self.scale)."""
+        torch = _torch()
+        n = len(rhos)
+        self.scale = float(np.std(targets)) or 1.0
+        tgt = torch.as_tensor(np.asarray(targets, dtype=np.float32) / self.scale, device="cuda")
+        rng = np.random.default_rng(seed)
+        hist = []
+        for ep in range(epochs):
+            order = rng.permutation(n)
+            tot = 0.0
+            for s in range(0, n - batch + 1, batch):
+                idx = order[s:s + batch]
+                out = self.forward(rhos[idx], keep=True)
+                diff = out - tgt[idx]
+                tot += float((diff * diff).mean()) * len(idx)
+                self.backward(2.0 * diff / diff.numel())
+                self.t += 1
+                for L in self.layers:
+                    L.adam(self.t, lr)
+            hist.append(tot / (n - n % batch))
+            if log:
+                log("epoch %d: train mse %.4e" % (ep, hist[-1]))
+        return hist
+
+    def predict(self, rho):
+        """Fields [B, 21, 21, 4] complex (numpy) for densities [B, 21, 101, 101]."""
+        out = self.forward(np.asarray(rho)[None] if np.ndim(rho) == 3 else rho).cpu().numpy() * self.scale
+        return out[..., 0::2] + 1j * out[..., 1::2]
+
+    def values(self, rhos, fidx=FREQS.index(0.1)):
+        """F = sum over the patch of |E(f)|^2 for a batch of densities (forward only)."""
+        out = self.forward(np.asarray(rhos), keep=False)
+        re, im = out[..., 2 * fidx], out[..., 2 * fidx + 1]
+        return ((re * re + im * im).sum(dim=(1, 2)) * self.scale ** 2).cpu().numpy().astype(np.float64)
+
+    def objective(self, rho, fidx=FREQS.index(0.1)):
+        """F = sum over the patch of |E(f)|^2 (the inverse-design objective) and
+        its gradient with respect to the density, from the network."""
+        torch = _torch()
+        out = self.forward(np.asarray(rho)[None], keep=True)
+        re, im = out[..., 2 * fidx], out[..., 2 * fidx + 1]
+        F = float(((re * re + im * im).sum()).item()) * self.scale ** 2
+        dout = torch.zeros_like(out)
+        dout[..., 2 * fidx] = 2 * re * self.scale ** 2
+        dout[..., 2 * fidx + 1] = 2 * im * self.scale ** 2
+        g = self.backward(dout, need_dx=True)
+        return F, g[0].cpu().numpy().astype(np.float64)
+
+
+# ---------------------------------------------------------------- data and loop
+def design_problem(steps=3000, precision="fp32"):
+    """The inverse-design workload (200x200x100, design box 100x100x20, eps 1..12,
+    F = sum |DFT Ez(0.1)|^2 on the 20x20 patch at z = +30) with an extra patch
+    monitor at the four surrogate frequencies."""
+    from .adjoint import c3_problem
+    from .geom import Vector3
+    p = c3_problem(steps=steps, precision=precision)
+    mon = p.fwd.add_dft_fields(center=Vector3(0, 0, 30.0), size=Vector3(20, 20, 0), frequency=list(FREQS),
+                               field_component="Ez")
+    return p, mon
+
+
+def patch_fields(mon):
+    xs = np.linspace(-10.0, 10.0, PATCH)
+    v = mon(np.array(FREQS), [30.0], xs, xs)[:, 0]  # [f, y, x]
+    return np.moveaxis(v, 0, -1)  # [y, x, f]
+
+
+def to_channels(fields):
+    """[..., f] complex -> [..., 2f] real (Re, Im interleaved)."""
+    out = np.empty(fields.shape[:-1] + (2 * fields.shape[-1],), dtype=np.float32)
+    out[..., 0::2] = fields.real
+    out[..., 1::2] = fields.imag
+    return out
+
+
+def generate(seeds, steps=3000, precision="fp32", log=None):
+    """Solve the forward problem for the blurred random densities of `seeds`;
+    returns (densities [N, 21, 101, 101], patch fields [N, 21, 21, 4])."""
+    from .workloads import blurred_density
+    p, mon = design_problem(steps, precision)
+    rhos, fields = [], []
+    t0 = time.time()
+    for q, s in enumerate(seeds):
+        rho = blurred_density(p.shape, seed=int(s), radius=3, sigma=1.5)
+        p.objective(rho)
+        rhos.append(rho.astype(np.float32))
+        fields.append(patch_fields(mon))
+        if log and (q + 1) % 32 == 0:
+            log("%d designs solved (%.1f s)" % (q + 1, time.time() - t0))
+    return np.stack(rhos), np.stack(fields)
+
+
+def ascent(problem, rho0, iters, surrogate=None, alpha0=0.05, inner=4, log=None):
+    """Normalised gradient ascent of F with full-solve acceptance.  Without a
+    surrogate, step lengths are found by backtracking with forward solves; with
+    one, `inner` surrogate-guided steps (step length chosen on the surrogate,
+    later directions from its gradient) produce the candidate, which one forward
+    solve accepts or rejects (rejection falls back to backtracking).  Returns the
+    design, F and the history [(full solves so far, F)]."""
+    solves = 0
+    F = problem.objective(rho0)
+    solves += 1
+    rho, alpha = np.asarray(rho0, dtype=np.float64), alpha0
+    hist = [(solves, F)]
+    accepted = 0
+    for it in range(iters):
+        g = problem.gradient()
+        solves += 1
+        d = g / (np.abs(g).max() + 1e-300)
+        cand = None
+        if surrogate is not None:
+            c = rho
+            step = d
+            for k in range(inner):
+                trials = np.stack([np.clip(c + a * step, 0, 1) for a in alpha * np.array([0.5, 1.0, 2.0, 4.0])])
+                c = trials[int(np.argmax(surrogate.values(trials.astype(np.float32))))]
+                if k + 1 < inner:
+                    gs = surrogate.objective(c)[1]
+                    step = gs / (np.abs(gs).max() + 1e-300)
+            Fc = problem.objective(c)
+            solves += 1
+            if Fc > F:
+                cand, F = c, Fc
+        while cand is None and alpha > 1e-4:
+            c = np.clip(rho + alpha * d, 0, 1)
+            Fc = problem.objective(c)
+            solves += 1
+            if Fc > F:
+                cand, F = c, Fc
+                alpha *= 1.5
+            else:
+                alpha *= 0.5
+        if cand is None:
+            break
+        rho = cand
+        accepted += 1
+        hist.append((solves, F))
+        if log:
+            log("iter %d: F = %.6e, full solves %d" % (it, F, solves))
+    return rho, F, hist

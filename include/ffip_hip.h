@@ -133,6 +133,59 @@ int ffip_adjoint_gradient(const ffip_sim *fwd, const ffip_sim *adj, const int fw
 int ffip_transpose(const double *gx, int64_t nx, const double *gy, int64_t ny, const double *gz, int64_t nz,
                    const double *pts, int64_t npts, double *out);
 
+/* ------------------------------------------------------------------------
+ * Design surrogate kernels.  No counterpart in the reference: a 3-D
+ * convolutional network over the design density, trained and evaluated on the
+ * matrix cores (BF16 operands, FP32 accumulation), proposes objective and
+ * gradient values for inner line-search steps; every design it proposes is
+ * re-judged by the full FDTD + adjoint solve.  All pointers are device pointers;
+ * `stream` is a hipStream_t (NULL = default stream); nothing is allocated and
+ * every call is asynchronous on that stream.  bf16 arrays hold IEEE bfloat16.
+ * ------------------------------------------------------------------------ */
+
+/* C (+)= op(A) op(B): op(A) M x K, op(B) K x N (bf16), C FP32 row-major (ldc).
+ * ta: A stored K x M (element (m, k) at A[k*lda + m]); tb: B stored N x K.
+ * accumulate: add to C instead of overwriting; ksplit > 1 splits K over
+ * workgroups (partial sums added atomically). */
+int ffip_sg_gemm(int ta, int tb, int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B,
+                 int64_t ldb, float *C, int64_t ldc, int accumulate, int ksplit, void *stream);
+
+/* 3x3x3 convolution taps of a channels-last bf16 activation [batch][z][y][x][C]
+ * with stride s and dilation d per axis (padding d, output extent
+ * (in - 1) / s + 1): col[(b, oz, oy, ox)][(kz, ky, kx, c)].
+ * geom = {batch, C, in_z, in_y, in_x, s_z, s_y, s_x, d_z, d_y, d_x}. */
+int ffip_sg_im2col(const int64_t geom[11], const void *X, void *col, void *stream);
+
+/* Adjoint of ffip_sg_im2col on FP32 data: dX (overwritten) = sum of the column
+ * entries that read each input element. */
+int ffip_sg_col2im(const int64_t geom[11], const float *dcol, float *dX, void *stream);
+
+/* out (bf16) = act(Y + bias) over rows x n (bias per column, act = ReLU if relu);
+ * zkeep (FP32, may be NULL) receives Y + bias. */
+int ffip_sg_bias_act(int64_t rows, int64_t n, const float *Y, const float *bias, int relu, void *out, float *zkeep,
+                     void *stream);
+
+/* dZ = relu ? dA * (Z > 0) : dA, written as bf16 (dZb) and FP32 (dZf). */
+int ffip_sg_act_back(int64_t total, const float *dA, const float *Z, int relu, void *dZb, float *dZf, void *stream);
+
+/* db[c] (+)= sum over rows of dZ[rows][n]. */
+int ffip_sg_colsum(int64_t rows, int64_t n, const float *dZ, float *db, int accumulate, void *stream);
+
+/* y (bf16) = x (FP32), n elements. */
+int ffip_sg_to_bf16(int64_t n, const float *x, void *y, void *stream);
+
+/* Network head: out[b][y][x][c] = mean over z of A[b][z][y0 + y][x0 + x][c]
+ * (A bf16 [batch][nz][ny_in][nx_in][C]); h = {batch, nz, ny_in, nx_in, C, y0, x0,
+ * ny, nx}.  ffip_sg_head_back writes dA (FP32, overwritten). */
+int ffip_sg_head(const int64_t h[9], const void *A, float *out, void *stream);
+int ffip_sg_head_back(const int64_t h[9], const float *dout, float *dA, void *stream);
+
+/* Adam step on FP32 weights w with gradient g and moments m, v; hp = {lr,
+ * beta1, beta2, eps}; t = step count from 1; wb (may be NULL) receives the
+ * updated weights as bf16. */
+int ffip_sg_adam(int64_t n, float *w, const float *g, float *m, float *v, const float hp[4], int64_t t, void *wb,
+                 void *stream);
+
 #ifdef __cplusplus
 }
 #endif
