@@ -57,6 +57,8 @@ _SIGS = {
     "ffip_transpose": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                         ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
     "ffip_sim_destroy": ([ctypes.c_void_p], None),
+    "ffip_to_convolve": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                          ctypes.c_int64, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     # surrogate kernels (device pointers, stream as void*)
     "ffip_sg_gemm": ([ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                       ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
@@ -120,6 +122,32 @@ def transpose(gx, gy, gz, pt):
     _check(lib().ffip_transpose(_ptr(gx), gx.size, _ptr(gy), gy.size, _ptr(gz), gz.size, _ptr(pt), pt.shape[0],
                                 _ptr(out)))
     return out
+
+
+def _to_convolve(in1, in2, transpose):
+    a = np.ascontiguousarray(np.asarray(in1, dtype=np.float64))
+    f = np.ascontiguousarray(np.asarray(in2, dtype=np.float64))
+    if a.ndim != 2 or f.ndim != 2:
+        raise KeyError("dimensions are not 2")
+    if f.shape[0] % 2 == 0 or f.shape[1] % 2 == 0:
+        raise KeyError("filter shape is not odd")
+    if f.shape[0] != f.shape[1]:
+        raise KeyError("filter is not square (the reference indexes both axes with the first radius)")
+    out = np.empty_like(a)
+    _check(lib().ffip_to_convolve(_ptr(a), a.shape[0], a.shape[1], _ptr(f), f.shape[0], f.shape[1], int(transpose),
+                                  _ptr(out)))
+    return out
+
+
+def TO_convolve(in1, in2):
+    """Window-normalised 2-D correlation of in1 with the odd square filter in2
+    (the reference's _ffip.TO_convolve, app/ffip.cpp:222-262), on the GPU."""
+    return _to_convolve(in1, in2, False)
+
+
+def TO_convolve_transpose(in1, in2):
+    """Adjoint of TO_convolve (app/ffip.cpp:264-306), on the GPU."""
+    return _to_convolve(in1, in2, True)
 
 
 def adjoint_gradient(fwd: "LibSim", adj: "LibSim", fwd_out, adj_out, freq, eps_prime, norm, center, size, dim):

@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libffip_hip.so")
 ARCH = os.environ.get("FFIP_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["json.cpp", "setup.cpp", "capi.cpp"]
-HIP_SOURCES = ["engine.hip", "surrogate.hip"]
+HIP_SOURCES = ["engine.hip", "surrogate.hip", "filters.hip"]
 
 
 def _hipcc():

@@ -13,6 +13,11 @@
 #include "setup.hpp"
 #include "surrogate.hpp"
 
+namespace ffip_amd {
+void to_convolve(const double* in, int64_t n0, int64_t n1, const double* filt, int64_t f0, int64_t f1, bool transpose,
+                 double* out);
+}
+
 using namespace ffip_amd;
 
 struct ffip_sim {
@@ -597,6 +602,14 @@ int ffip_transpose(const double* gx, int64_t nx, const double* gy, int64_t ny, c
             transpose_axis(out, dims, strides, q, 3, p[3]);
         }
     });
+}
+
+int ffip_to_convolve(const double* in, int64_t n0, int64_t n1, const double* filt, int64_t f0, int64_t f1,
+                     int transpose, double* out)
+{
+    if (!in || !filt || !out)
+        return fail(FFIP_EINVAL, "null argument");
+    return guard([&] { to_convolve(in, n0, n1, filt, f0, f1, transpose != 0, out); });
 }
 
 // ------------------------------------------------------------ surrogate kernels

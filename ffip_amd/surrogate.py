@@ -327,3 +327,96 @@ def ascent(problem, rho0, iters, surrogate=None, alpha0=0.05, inner=4, log=None)
         if log:
             log("iter %d: F = %.6e, full solves %d" % (it, F, solves))
     return rho, F, hist
+
+
+# ---------------------------------------------------------------- experiment
+def main(argv=None):
+    """Surrogate-assisted design experiment on the inverse-design workload.
+
+        python -m ffip_amd.surrogate data  --out FIELDS.npz [--designs 512]
+        python -m ffip_amd.surrogate study --data FIELDS.npz --out RESULT.json [--iters 25]
+
+    `data` solves the forward problem for the blurred random densities of seeds
+    1000.. (FP32 storage) and saves the patch fields; `study` trains the network
+    on all but the last 64 designs, reports its held-out error, then runs the
+    plain and the surrogate-assisted gradient ascent from the same start and
+    counts full solves."""
+    import argparse
+    import json
+    import sys
+
+    ap = argparse.ArgumentParser(prog="python -m ffip_amd.surrogate")
+    ap.add_argument("stage", choices=["data", "study"])
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--data")
+    ap.add_argument("--designs", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--iters", type=int, default=25)
+    ap.add_argument("--inner", type=int, default=4)
+    ap.add_argument("--heldout", type=int, default=64)
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print("[surrogate] " + msg, flush=True)
+
+    from .workloads import blurred_density
+    if a.stage == "data":
+        seeds = np.arange(1000, 1000 + a.designs)
+        rhos, fields = generate(seeds, steps=a.steps, log=log)
+        np.savez(a.out, seeds=seeds, fields=fields, steps=a.steps)
+        log("saved %s" % a.out)
+        return 0
+    d = np.load(a.data)
+    seeds, fields = d["seeds"], d["fields"]
+    shape = (21, 101, 101)
+    rhos = np.stack([blurred_density(shape, seed=int(s), radius=3, sigma=1.5) for s in seeds]).astype(np.float32)
+    ntr = len(seeds) - a.heldout
+    net = Surrogate(seed=0)
+    t0 = time.time()
+    hist = net.fit(rhos[:ntr], to_channels(fields[:ntr]), epochs=a.epochs, log=log)
+    t_train = time.time() - t0
+    pred = np.concatenate([net.predict(rhos[q:q + 8]) for q in range(ntr, len(seeds), 8)])
+    true = fields[ntr:]
+    field_err = float(np.linalg.norm(pred - true) / np.linalg.norm(true))
+    fi = FREQS.index(0.1)
+    F_true = (np.abs(true[..., fi]) ** 2).sum(axis=(1, 2))
+    F_pred = (np.abs(pred[..., fi]) ** 2).sum(axis=(1, 2))
+    F_err = float(np.median(np.abs(F_pred - F_true) / F_true))
+    corr = float(np.corrcoef(F_pred, F_true)[0, 1])
+    log("held-out: field rel L2 %.3f, objective median rel err %.3f, corr %.3f (train %.1f s)"
+        % (field_err, F_err, corr, t_train))
+
+    from .adjoint import c3_problem
+    rho0 = blurred_density(shape, seed=2000, radius=3, sigma=1.5)
+    runs = {}
+    for name, sg in (("plain", None), ("surrogate", net)):
+        p = c3_problem(steps=a.steps, precision="fp32")
+        t0 = time.time()
+        rho, F, h = ascent(p, rho0, a.iters, surrogate=sg, inner=a.inner, log=lambda m: log(name + " " + m))
+        runs[name] = {"final_F": F, "history": h, "seconds": time.time() - t0}
+    target = runs["plain"]["final_F"]
+    for name, r in runs.items():
+        h = r["history"]
+        reach = next((s for s, F in h if F >= target), None)
+        r["solves_to_plain_final"] = reach
+        r["accepted_steps"] = len(h) - 1
+        r["total_solves"] = h[-1][0]
+    res = {"train_mse": hist, "train_seconds": t_train, "heldout_field_rel_l2": field_err,
+           "heldout_objective_median_rel_err": F_err, "heldout_objective_corr": corr,
+           "designs": int(len(seeds)), "train_designs": int(ntr), "iters": a.iters, "inner": a.inner, "runs": runs}
+    if runs["surrogate"]["solves_to_plain_final"]:
+        res["solve_ratio_at_plain_final"] = runs["plain"]["solves_to_plain_final"] / runs["surrogate"][
+            "solves_to_plain_final"]
+    with open(a.out, "w") as f:
+        json.dump(res, f, indent=1, default=float)
+    log(json.dumps({k: v for k, v in res.items() if k not in ("train_mse", "runs")}, default=float))
+    for name, r in runs.items():
+        log("%s: final F %.6e, %d accepted steps, %d full solves, %s solves to reach the plain run's final F"
+            % (name, r["final_F"], r["accepted_steps"], r["total_solves"], r["solves_to_plain_final"]))
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())

@@ -133,6 +133,16 @@ int ffip_adjoint_gradient(const ffip_sim *fwd, const ffip_sim *adj, const int fw
 int ffip_transpose(const double *gx, int64_t nx, const double *gy, int64_t ny, const double *gz, int64_t nz,
                    const double *pts, int64_t npts, double *out);
 
+/* Density filter of filtered-design scripts (replaces the pybind helpers
+ * _ffip.TO_convolve / TO_convolve_transpose, app/ffip.cpp:222-306): in and out
+ * are host arrays [n0][n1], filt [f0][f1] with odd, equal f0 and f1 (the
+ * reference indexes both filter axes with the first radius, so non-square
+ * filters read outside it there; rejected here).  transpose = 0: out(i, j) =
+ * window-normalised correlation of in with filt; 1: its adjoint.  Computed on
+ * the GPU in the reference's summation order. */
+int ffip_to_convolve(const double *in, int64_t n0, int64_t n1, const double *filt, int64_t f0, int64_t f1,
+                     int transpose, double *out);
+
 /* ------------------------------------------------------------------------
  * Design surrogate kernels.  No counterpart in the reference: a 3-D
  * convolutional network over the design density, trained and evaluated on the

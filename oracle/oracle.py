@@ -1086,3 +1086,50 @@ class OracleSim:
         X = yee_coords(p1, p2, 2)
         dims = [(p2[a] - p1[a]) // 2 + 1 for a in range(3)]
         return self.eh[self.grid.index(X[:, 0], X[:, 1], X[:, 2])].reshape(dims[2], dims[1], dims[0])
+
+
+# ----------------------------------------------------------------------------
+# Density filters of filtered-design scripts (app/ffip.cpp:222-306)
+# ----------------------------------------------------------------------------
+
+def to_convolve(p1, p2):
+    """TO_convolve, app/ffip.cpp:222-262: window-normalised correlation; loops and
+    summation order as there (both filter axes indexed with the first radius,
+    app/ffip.cpp:248, so only square filters are defined)."""
+    p1 = np.asarray(p1, dtype=float)
+    p2 = np.asarray(p2, dtype=float)
+    r0, r1 = p2.shape[0] // 2, p2.shape[1] // 2
+    n0, n1 = p1.shape
+    out = np.zeros_like(p1)
+    for i in range(n0):
+        for j in range(n1):
+            w = 0.0
+            s = 0.0
+            for m in range(max(i - r0, 0), min(i + r0 + 1, n0)):
+                for n in range(max(j - r1, 0), min(j + r1 + 1, n1)):
+                    w += p2[m - i + r0, n - j + r0]
+                    s += p1[m, n] * p2[m - i + r0, n - j + r0]
+            out[i, j] = s / w
+    return out
+
+
+def to_convolve_transpose(p1, p2):
+    """TO_convolve_transpose, app/ffip.cpp:264-306: the scatter of p1(i, j) times
+    the filter over the window of (i, j), divided by that window's weight."""
+    p1 = np.asarray(p1, dtype=float)
+    p2 = np.asarray(p2, dtype=float)
+    r0, r1 = p2.shape[0] // 2, p2.shape[1] // 2
+    n0, n1 = p1.shape
+    out = np.zeros_like(p1)
+    for i in range(n0):
+        for j in range(n1):
+            lo0, lo1 = max(i - r0, 0), max(j - r1, 0)
+            hi0, hi1 = min(i + r0 + 1, n0), min(j + r1 + 1, n1)
+            w = 0.0
+            for m in range(lo0, hi0):
+                for n in range(lo1, hi1):
+                    w += p2[m - i + r0, n - j + r0]
+            for m in range(lo0, hi0):
+                for n in range(lo1, hi1):
+                    out[m, n] += p1[i, j] * p2[m - i + r0, n - j + r0] / w
+    return out
