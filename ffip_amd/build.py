@@ -17,10 +17,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libffip_hip.so")
+EXE = os.path.join(HERE, "run_sim_json")
 ARCH = os.environ.get("FFIP_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["json.cpp", "setup.cpp", "capi.cpp"]
 HIP_SOURCES = ["engine.hip", "surrogate.hip", "filters.hip"]
+# the file drop-in executable (links the library; libhdf5 is loaded at run time)
+EXE_SOURCES = ["run_sim_json.cpp", "hdf5_io.cpp", "json.cpp"]
 
 
 def _hipcc():
@@ -55,10 +58,20 @@ def build(verbose: bool = False) -> str:
               "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-Wl,-soname,libffip_hip.so", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    exe_objs = []
+    for src in EXE_SOURCES:
+        obj = os.path.join(BUILD, "exe_" + src + ".o")
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-c", os.path.join(CSRC, src), "-o", obj])
+        exe_objs.append(obj)
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc))), "lib")
+    tmp = EXE + ".tmp"
+    _run(["g++", "-o", tmp] + exe_objs + [LIB, "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link," + rocm_lib, "-ldl"])
+    os.replace(tmp, EXE)
     if verbose:
         print("built", LIB)
+        print("built", EXE)
     return LIB
 
 

@@ -113,8 +113,9 @@ struct HalfArgs {
     //  full grid: pole s of component c at cell o is element (s*3 + c)*pslice + o;
     //  compact:   only points with poles hold entries; component c's start at
     //             pcomp[c], the row segment (k, j, x tile) holds its points' poles
-    //             point-major (in x order) from segbase[c][((k - dlo_z) * seg_ny +
-    //             (j - dlo_y)) * gx + tx].
+    //             from segbase[c][((k - dlo_z) * seg_ny + (j - dlo_y)) * gx + tx],
+    //             pole-major: pole 0 of the segment's points (in x order) that
+    //             have it, then pole 1, and so on.
     T* pcur;
     T* pprev;
     int64_t pslice;
@@ -1054,11 +1055,11 @@ struct EngineImpl {
 EngineImpl::~EngineImpl()
 {
     if (stream)
-        hipStreamSynchronize(stream);
+        (void)hipStreamSynchronize(stream);
     for (void* p : allocs)
-        hipFree(p);
+        (void)hipFree(p);
     if (stream)
-        hipStreamDestroy(stream);
+        (void)hipStreamDestroy(stream);
 }
 
 PolLayoutPlan plan_polarization(const Setup& s, int h)
@@ -1738,10 +1739,10 @@ KernelTiming Engine::step_timed(int64_t nsteps, bool per_kernel)
             r.h_ms += b;
         }
         for (auto& e : ev)
-            hipEventDestroy(e);
+            (void)hipEventDestroy(e);
     }
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
     return r;
 }
 

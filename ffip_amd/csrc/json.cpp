@@ -1,6 +1,7 @@
 #include "json.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
@@ -256,6 +257,75 @@ double Json::num_or(const std::string& key, double dflt) const
 {
     const Json* j = find(key);
     return j ? j->num() : dflt;
+}
+
+static void dump_string(const std::string& v, std::string& out)
+{
+    out += '"';
+    for (unsigned char c : v) {
+        if (c == '"' || c == '\\') {
+            out += '\\';
+            out += char(c);
+        } else if (c < 0x20) {
+            char buf[8];
+            std::snprintf(buf, sizeof buf, "\\u%04x", c);
+            out += buf;
+        } else {
+            out += char(c);
+        }
+    }
+    out += '"';
+}
+
+static void dump_value(const Json& j, std::string& out)
+{
+    switch (j.kind()) {
+    case Json::Null:
+        out += "null";
+        break;
+    case Json::Bool:
+        out += j.boolean() ? "true" : "false";
+        break;
+    case Json::Number: {
+        double v = j.num();
+        if (!std::isfinite(v))
+            throw std::runtime_error("JSON cannot hold a non-finite number");
+        char buf[32];
+        std::snprintf(buf, sizeof buf, "%.17g", v);
+        out += buf;
+        break;
+    }
+    case Json::String:
+        dump_string(j.str(), out);
+        break;
+    case Json::Array:
+        out += '[';
+        for (size_t i = 0; i < j.elems().size(); ++i) {
+            if (i)
+                out += ',';
+            dump_value(j.elems()[i], out);
+        }
+        out += ']';
+        break;
+    case Json::Object:
+        out += '{';
+        for (size_t i = 0; i < j.items().size(); ++i) {
+            if (i)
+                out += ',';
+            dump_string(j.items()[i].first, out);
+            out += ':';
+            dump_value(j.items()[i].second, out);
+        }
+        out += '}';
+        break;
+    }
+}
+
+std::string Json::dump() const
+{
+    std::string out;
+    dump_value(*this, out);
+    return out;
 }
 
 }  // namespace ffip_amd

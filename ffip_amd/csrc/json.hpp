@@ -42,6 +42,10 @@ public:
 
     double num_or(const std::string& key, double dflt) const;
 
+    // Serialises the value (numbers with 17 significant digits, so they parse
+    // back to the same doubles).
+    std::string dump() const;
+
 private:
     Kind kind_ = Null;
     double num_ = 0;

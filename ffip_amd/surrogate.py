@@ -2,17 +2,17 @@
 
 No counterpart in the reference.  A small 3-D convolutional network maps the
 design density of the inverse-design workload (21 x 101 x 101 density nodes of a
-100 x 100 x 20-cell two-medium box) to the complex Ez DFT on the 21 x 21 objective
-patch at four frequencies.  It is trained on (density, patch DFT) pairs that the
+100 x 100 x 20-cell two-medium box) to the Ez DFT intensity on the 21 x 21
+objective patch at four frequencies.  It is trained on (density, patch DFT) pairs that the
 GPU FDTD solver produces, and it proposes step lengths and inner steps of an
 adjoint gradient ascent; every design it proposes is re-judged by the full FDTD
 solve (and the adjoint solve when accepted), which stays the fallback.
 
-Network: four 3x3x3 convolutions (channels 1 -> 32 -> 32 -> 32 -> 8), each
+Network: four 3x3x3 convolutions (channels 1 -> 32 -> 32 -> 32 -> 4), each
 striding 2 along z (21 -> 11 -> 6 -> 3 -> 2 planes) with x/y dilations 1, 2, 4, 8
 (receptive field +-15 cells), ReLU after the first three; the head averages over
-z and crops the 21 x 21 patch; the 8 output channels are (Re, Im) at 4
-frequencies.  Each convolution runs as im2col + a BF16 MFMA GEMM with FP32
+z and crops the 21 x 21 patch; the 4 output channels are the intensities
+|E_f|^2 at the 4 frequencies (the objective is their sum at 0.1).  Each convolution runs as im2col + a BF16 MFMA GEMM with FP32
 accumulation (ffip_sg_*), its backward pass as two more GEMMs + col2im; Adam
 updates FP32 master weights and refreshes their bf16 copies.  torch only holds
 the device memory and supplies the stream.
@@ -136,9 +136,9 @@ class Conv3:
 
 
 class Surrogate:
-    """density [B, 21, 101, 101] in [0, 1] -> patch fields [B, 21, 21, 8] (Re, Im x 4 freqs)."""
+    """density [B, 21, 101, 101] in [0, 1] -> patch intensities |E_f|^2 [B, 21, 21, 4]."""
 
-    def __init__(self, seed=0, shape=(21, 101, 101), crop=(40, 40, PATCH, PATCH), width=32, nout=2 * len(FREQS)):
+    def __init__(self, seed=0, shape=(21, 101, 101), crop=(40, 40, PATCH, PATCH), width=32, nout=len(FREQS)):
         rng = np.random.default_rng(seed)
         self.shape, self.crop, self.nout = tuple(shape), crop, nout
         dil = [(1, 1, 1), (1, 2, 2), (1, 4, 4), (1, 8, 8)]
@@ -185,52 +185,62 @@ class Surrogate:
         return None
 
     # ----------------------------------------------------------- training
-    def fit(self, rhos, targets, epochs=30, batch=8, lr=1e-3, seed=0, log=None):
-        """Mean-squared error on targets [N, 21, 21, 8] (fields divided by self.scale)."""
+    def _steps(self, rhos, tgt, order, batch, lr):
+        tot, cnt = 0.0, 0
+        for s in range(0, len(order) - batch + 1, batch):
+            idx = order[s:s + batch]
+            out = self.forward(rhos[idx], keep=True)
+            diff = out - tgt[idx]
+            tot += float((diff * diff).mean()) * len(idx)
+            cnt += len(idx)
+            self.backward(2.0 * diff / diff.numel())
+            self.t += 1
+            for L in self.layers:
+                L.adam(self.t, lr)
+        return tot / max(cnt, 1)
+
+    def fit(self, rhos, targets, epochs=60, batch=8, lr=2e-3, seed=0, log=None):
+        """Mean-squared error on targets [N, 21, 21, nout] divided by self.scale
+        (set here from the targets and kept for later fine-tuning)."""
         torch = _torch()
-        n = len(rhos)
-        self.scale = float(np.std(targets)) or 1.0
+        self.scale = float(np.mean(np.abs(targets))) or 1.0
         tgt = torch.as_tensor(np.asarray(targets, dtype=np.float32) / self.scale, device="cuda")
         rng = np.random.default_rng(seed)
         hist = []
         for ep in range(epochs):
-            order = rng.permutation(n)
-            tot = 0.0
-            for s in range(0, n - batch + 1, batch):
-                idx = order[s:s + batch]
-                out = self.forward(rhos[idx], keep=True)
-                diff = out - tgt[idx]
-                tot += float((diff * diff).mean()) * len(idx)
-                self.backward(2.0 * diff / diff.numel())
-                self.t += 1
-                for L in self.layers:
-                    L.adam(self.t, lr)
-            hist.append(tot / (n - n % batch))
+            hist.append(self._steps(rhos, tgt, rng.permutation(len(rhos)), batch, lr))
             if log:
                 log("epoch %d: train mse %.4e" % (ep, hist[-1]))
         return hist
 
+    def fine_tune(self, rhos, targets, steps=20, batch=4, lr=1e-3, seed=0):
+        """A few Adam steps on a small set of (density, target) pairs (online
+        updates from the solves of an optimisation run)."""
+        torch = _torch()
+        tgt = torch.as_tensor(np.asarray(targets, dtype=np.float32) / self.scale, device="cuda")
+        rng = np.random.default_rng(seed + self.t)
+        n = len(rhos)
+        b = min(batch, n)
+        for _ in range(steps):
+            self._steps(rhos, tgt, rng.choice(n, size=b, replace=False), b, lr)
+
     def predict(self, rho):
-        """Fields [B, 21, 21, 4] complex (numpy) for densities [B, 21, 101, 101]."""
-        out = self.forward(np.asarray(rho)[None] if np.ndim(rho) == 3 else rho).cpu().numpy() * self.scale
-        return out[..., 0::2] + 1j * out[..., 1::2]
+        """Patch intensities |E_f|^2 [B, 21, 21, 4] (numpy) for densities [B, 21, 101, 101]."""
+        return self.forward(np.asarray(rho)[None] if np.ndim(rho) == 3 else rho).cpu().numpy() * self.scale
 
     def values(self, rhos, fidx=FREQS.index(0.1)):
         """F = sum over the patch of |E(f)|^2 for a batch of densities (forward only)."""
         out = self.forward(np.asarray(rhos), keep=False)
-        re, im = out[..., 2 * fidx], out[..., 2 * fidx + 1]
-        return ((re * re + im * im).sum(dim=(1, 2)) * self.scale ** 2).cpu().numpy().astype(np.float64)
+        return (out[..., fidx].sum(dim=(1, 2)) * self.scale).cpu().numpy().astype(np.float64)
 
     def objective(self, rho, fidx=FREQS.index(0.1)):
         """F = sum over the patch of |E(f)|^2 (the inverse-design objective) and
         its gradient with respect to the density, from the network."""
         torch = _torch()
         out = self.forward(np.asarray(rho)[None], keep=True)
-        re, im = out[..., 2 * fidx], out[..., 2 * fidx + 1]
-        F = float(((re * re + im * im).sum()).item()) * self.scale ** 2
+        F = float(out[..., fidx].sum().item()) * self.scale
         dout = torch.zeros_like(out)
-        dout[..., 2 * fidx] = 2 * re * self.scale ** 2
-        dout[..., 2 * fidx + 1] = 2 * im * self.scale ** 2
+        dout[..., fidx] = self.scale
         g = self.backward(dout, need_dx=True)
         return F, g[0].cpu().numpy().astype(np.float64)
 
@@ -254,12 +264,9 @@ def patch_fields(mon):
     return np.moveaxis(v, 0, -1)  # [y, x, f]
 
 
-def to_channels(fields):
-    """[..., f] complex -> [..., 2f] real (Re, Im interleaved)."""
-    out = np.empty(fields.shape[:-1] + (2 * fields.shape[-1],), dtype=np.float32)
-    out[..., 0::2] = fields.real
-    out[..., 1::2] = fields.imag
-    return out
+def to_intensity(fields):
+    """[..., f] complex patch fields -> [..., f] intensities |E|^2 (the network's targets)."""
+    return (np.abs(fields) ** 2).astype(np.float32)
 
 
 def generate(seeds, steps=3000, precision="fp32", log=None):
@@ -279,54 +286,74 @@ def generate(seeds, steps=3000, precision="fp32", log=None):
     return np.stack(rhos), np.stack(fields)
 
 
-def ascent(problem, rho0, iters, surrogate=None, alpha0=0.05, inner=4, log=None):
-    """Normalised gradient ascent of F with full-solve acceptance.  Without a
-    surrogate, step lengths are found by backtracking with forward solves; with
-    one, `inner` surrogate-guided steps (step length chosen on the surrogate,
-    later directions from its gradient) produce the candidate, which one forward
-    solve accepts or rejects (rejection falls back to backtracking).  Returns the
-    design, F and the history [(full solves so far, F)]."""
-    solves = 0
-    F = problem.objective(rho0)
-    solves += 1
+def ascent(problem, rho0, iters, surrogate=None, monitor=None, alpha0=0.05, inner=1, tune_steps=20, log=None):
+    """Normalised projected-gradient ascent of F on [0, 1] with full-solve
+    acceptance.  Without a surrogate, step lengths are found by backtracking
+    with forward solves.  With one, `inner` surrogate-guided steps (step length
+    chosen on the network, later directions from its gradient) produce the
+    candidate, which one forward solve accepts or rejects (rejection falls back
+    to backtracking); after every forward solve the network is fine-tuned on
+    the run's latest solves (the patch `monitor` supplies their targets).  The run stops early when
+    backtracking finds no ascent step longer than 1e-4.  Returns the design, F,
+    the history [(full solves so far, F)] of accepted steps and the total number
+    of full solves (forward and adjoint), including those of a final failed
+    search."""
+    count = [0]
+    seen_r, seen_t = [], []
+
+    def forward(c):
+        F = problem.objective(c)
+        count[0] += 1
+        if surrogate is not None and monitor is not None:
+            seen_r.append(np.asarray(c, dtype=np.float32))
+            seen_t.append(to_intensity(patch_fields(monitor)))
+            surrogate.fine_tune(np.stack(seen_r[-16:]), np.stack(seen_t[-16:]), steps=tune_steps)
+        return F
+
+    F = forward(rho0)
     rho, alpha = np.asarray(rho0, dtype=np.float64), alpha0
-    hist = [(solves, F)]
-    accepted = 0
+    hist = [(count[0], F)]
     for it in range(iters):
         g = problem.gradient()
-        solves += 1
+        count[0] += 1
+        # projected gradient: entries pushing a bound-held density outwards are
+        # dropped before normalising, so the step moves the free densities
+        g = np.where(((rho >= 1) & (g > 0)) | ((rho <= 0) & (g < 0)), 0.0, g)
         d = g / (np.abs(g).max() + 1e-300)
         cand = None
         if surrogate is not None:
-            c = rho
-            step = d
+            c, step, first = rho, d, None
             for k in range(inner):
-                trials = np.stack([np.clip(c + a * step, 0, 1) for a in alpha * np.array([0.5, 1.0, 2.0, 4.0])])
-                c = trials[int(np.argmax(surrogate.values(trials.astype(np.float32))))]
+                lens = alpha * np.array([0.5, 1.0, 2.0, 4.0])
+                trials = np.stack([np.clip(c + a * step, 0, 1) for a in lens])
+                best = int(np.argmax(surrogate.values(trials.astype(np.float32))))
+                c = trials[best]
+                first = lens[best] if first is None else first
                 if k + 1 < inner:
                     gs = surrogate.objective(c)[1]
                     step = gs / (np.abs(gs).max() + 1e-300)
-            Fc = problem.objective(c)
-            solves += 1
+            Fc = forward(c)
             if Fc > F:
-                cand, F = c, Fc
+                # the network may lengthen the step, never shorten it; success
+                # grows it as in the plain run
+                cand, F, alpha = c, Fc, max(alpha, first) * 1.5
         while cand is None and alpha > 1e-4:
             c = np.clip(rho + alpha * d, 0, 1)
-            Fc = problem.objective(c)
-            solves += 1
+            Fc = forward(c)
             if Fc > F:
                 cand, F = c, Fc
                 alpha *= 1.5
             else:
                 alpha *= 0.5
         if cand is None:
+            if log:
+                log("iter %d: no ascent step longer than 1e-4, stopping (full solves %d)" % (it, count[0]))
             break
         rho = cand
-        accepted += 1
-        hist.append((solves, F))
+        hist.append((count[0], F))
         if log:
-            log("iter %d: F = %.6e, full solves %d" % (it, F, solves))
-    return rho, F, hist
+            log("iter %d: F = %.6e, full solves %d" % (it, F, count[0]))
+    return rho, F, hist, count[0]
 
 
 # ---------------------------------------------------------------- experiment
@@ -351,9 +378,10 @@ def main(argv=None):
     ap.add_argument("--data")
     ap.add_argument("--designs", type=int, default=512)
     ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=100)
     ap.add_argument("--iters", type=int, default=25)
-    ap.add_argument("--inner", type=int, default=4)
+    ap.add_argument("--inner", type=int, default=1)
+    ap.add_argument("--tune-steps", type=int, default=20)
     ap.add_argument("--heldout", type=int, default=64)
     a = ap.parse_args(argv)
 
@@ -374,37 +402,37 @@ def main(argv=None):
     ntr = len(seeds) - a.heldout
     net = Surrogate(seed=0)
     t0 = time.time()
-    hist = net.fit(rhos[:ntr], to_channels(fields[:ntr]), epochs=a.epochs, log=log)
+    hist = net.fit(rhos[:ntr], to_intensity(fields[:ntr]), epochs=a.epochs, log=log)
     t_train = time.time() - t0
     pred = np.concatenate([net.predict(rhos[q:q + 8]) for q in range(ntr, len(seeds), 8)])
-    true = fields[ntr:]
+    true = to_intensity(fields[ntr:])
     field_err = float(np.linalg.norm(pred - true) / np.linalg.norm(true))
     fi = FREQS.index(0.1)
-    F_true = (np.abs(true[..., fi]) ** 2).sum(axis=(1, 2))
-    F_pred = (np.abs(pred[..., fi]) ** 2).sum(axis=(1, 2))
+    F_true = true[..., fi].sum(axis=(1, 2))
+    F_pred = pred[..., fi].sum(axis=(1, 2))
     F_err = float(np.median(np.abs(F_pred - F_true) / F_true))
     corr = float(np.corrcoef(F_pred, F_true)[0, 1])
-    log("held-out: field rel L2 %.3f, objective median rel err %.3f, corr %.3f (train %.1f s)"
+    log("held-out: intensity rel L2 %.3f, objective median rel err %.3f, corr %.3f (train %.1f s)"
         % (field_err, F_err, corr, t_train))
 
-    from .adjoint import c3_problem
     rho0 = blurred_density(shape, seed=2000, radius=3, sigma=1.5)
     runs = {}
     for name, sg in (("plain", None), ("surrogate", net)):
-        p = c3_problem(steps=a.steps, precision="fp32")
+        p, mon = design_problem(a.steps, "fp32")
         t0 = time.time()
-        rho, F, h = ascent(p, rho0, a.iters, surrogate=sg, inner=a.inner, log=lambda m: log(name + " " + m))
-        runs[name] = {"final_F": F, "history": h, "seconds": time.time() - t0}
+        rho, F, h, total = ascent(p, rho0, a.iters, surrogate=sg, monitor=mon, inner=a.inner,
+                                  tune_steps=a.tune_steps, log=lambda m: log(name + " " + m))
+        runs[name] = {"final_F": F, "history": h, "total_solves": total, "seconds": time.time() - t0}
     target = runs["plain"]["final_F"]
     for name, r in runs.items():
         h = r["history"]
         reach = next((s for s, F in h if F >= target), None)
         r["solves_to_plain_final"] = reach
         r["accepted_steps"] = len(h) - 1
-        r["total_solves"] = h[-1][0]
-    res = {"train_mse": hist, "train_seconds": t_train, "heldout_field_rel_l2": field_err,
+    res = {"train_mse": hist, "train_seconds": t_train, "heldout_intensity_rel_l2": field_err,
            "heldout_objective_median_rel_err": F_err, "heldout_objective_corr": corr,
-           "designs": int(len(seeds)), "train_designs": int(ntr), "iters": a.iters, "inner": a.inner, "runs": runs}
+           "designs": int(len(seeds)), "train_designs": int(ntr), "iters": a.iters, "inner": a.inner,
+           "tune_steps": a.tune_steps, "runs": runs}
     if runs["surrogate"]["solves_to_plain_final"]:
         res["solve_ratio_at_plain_final"] = runs["plain"]["solves_to_plain_final"] / runs["surrogate"][
             "solves_to_plain_final"]
