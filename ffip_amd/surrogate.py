@@ -3,8 +3,8 @@
 No counterpart in the reference.  A small 3-D convolutional network maps the
 design density of the inverse-design workload (21 x 101 x 101 density nodes of a
 100 x 100 x 20-cell two-medium box) to the Ez DFT intensity on the 21 x 21
-objective patch at four frequencies.  It is trained on (density, patch DFT) pairs that the
-GPU FDTD solver produces, and it proposes step lengths and inner steps of an
+objective patch at four frequencies.  It is trained on (density, patch DFT)
+pairs that the GPU FDTD solver produces, and it chooses the step lengths of an
 adjoint gradient ascent; every design it proposes is re-judged by the full FDTD
 solve (and the adjoint solve when accepted), which stays the fallback.
 
@@ -12,7 +12,8 @@ Network: four 3x3x3 convolutions (channels 1 -> 32 -> 32 -> 32 -> 4), each
 striding 2 along z (21 -> 11 -> 6 -> 3 -> 2 planes) with x/y dilations 1, 2, 4, 8
 (receptive field +-15 cells), ReLU after the first three; the head averages over
 z and crops the 21 x 21 patch; the 4 output channels are the intensities
-|E_f|^2 at the 4 frequencies (the objective is their sum at 0.1).  Each convolution runs as im2col + a BF16 MFMA GEMM with FP32
+|E_f|^2 at the 4 frequencies (the objective is the patch sum of the 0.1
+channel).  Each convolution runs as im2col + a BF16 MFMA GEMM with FP32
 accumulation (ffip_sg_*), its backward pass as two more GEMMs + col2im; Adam
 updates FP32 master weights and refreshes their bf16 copies.  torch only holds
 the device memory and supplies the stream.
