@@ -18,6 +18,7 @@ hot loop) on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import contextlib
 import json
 import os
 import platform
@@ -162,7 +163,10 @@ def main():
               "setup_s": round(t_setup, 2)}
     sim.close()
 
-    grad = None if args.no_gradient else design_gradient(args.gradient_steps)
+    # the front end announces each Simulation on stdout, as the reference does;
+    # this program's stdout carries only the result line
+    with contextlib.redirect_stdout(sys.stderr):
+        grad = None if args.no_gradient else design_gradient(args.gradient_steps)
 
     cpu = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:

@@ -165,6 +165,30 @@ def adjoint_gradient(fwd: "LibSim", adj: "LibSim", fwd_out, adj_out, freq, eps_p
     return out
 
 
+def geometry_info(config, datasets: Optional[Dict[str, np.ndarray]] = None):
+    """Host-only answers to the configuration's geometry info requests
+    (ffip_sim_get_geom_info before finalize; needs no GPU): [(x, y, z), ...]."""
+    L = lib()
+    text = config if isinstance(config, str) else json.dumps(config)
+    cfg = json.loads(text)
+    h = ctypes.c_void_p()
+    _check(L.ffip_sim_create(text.encode(), ctypes.byref(h)))
+    try:
+        for name, arr in (datasets or {}).items():
+            a = np.ascontiguousarray(np.asarray(arr, dtype=np.float64).ravel())
+            _check(L.ffip_sim_set_dataset(h, name.encode(), _ptr(a), a.size))
+        out = []
+        for k in range(len(cfg.get("geometry info request", []))):
+            n = ctypes.c_int64()
+            _check(L.ffip_sim_geom_info_count(h, k, ctypes.byref(n)))
+            x, y, z = np.zeros(n.value), np.zeros(n.value), np.zeros(n.value)
+            _check(L.ffip_sim_get_geom_info(h, k, _ptr(x), _ptr(y), _ptr(z)))
+            out.append((x, y, z))
+        return out
+    finally:
+        L.ffip_sim_destroy(h)
+
+
 def plan(config, datasets: Optional[Dict[str, np.ndarray]] = None, probes: Sequence = ()):
     """Host-only set-up summary (ffip_sim_plan); needs no GPU."""
     L = lib()

@@ -47,7 +47,7 @@ def build(verbose: bool = False) -> str:
     objs = []
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        _run(["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+        _run(["g++", "-std=c++17", "-O2", "-fPIC", "-ffp-contract=off", "-pthread", "-Wall", "-Wextra",
               "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     for src in HIP_SOURCES:

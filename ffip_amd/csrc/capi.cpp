@@ -26,6 +26,8 @@ struct ffip_sim {
     std::vector<ProbeSpec> probes;
     std::unique_ptr<Engine> engine;
     int64_t trace_cap = 0;
+    // geometry info coordinates of a finalized handle, computed at the first query
+    std::vector<std::vector<double>> geom_cache[3];
 };
 
 static thread_local std::string g_err;
@@ -251,16 +253,50 @@ int ffip_sim_get_component(ffip_sim* sim, const char* component, double* out)
     });
 }
 
+// The set-up of a finalized handle, or else one built on the host for this call
+// (geometry info needs no device state).
+static const Setup& setup_for_query(const ffip_sim* sim, std::unique_ptr<Setup>& tmp)
+{
+    if (!sim)
+        throw std::invalid_argument("null handle");
+    if (sim->engine)
+        return sim->engine->setup();
+    tmp = std::make_unique<Setup>();
+    build_setup(sim->config, sim->datasets, sim->probes, *tmp);
+    return *tmp;
+}
+
+// Coordinates of geometry info request k: cached on a finalized handle (its
+// set-up no longer changes), recomputed on the host before that.
+static void geom_coords(const ffip_sim* sim, int k, std::vector<double>& x, std::vector<double>& y,
+                        std::vector<double>& z)
+{
+    std::unique_ptr<Setup> tmp;
+    const Setup& S = setup_for_query(sim, tmp);
+    if (k < 0 || k >= int(S.geom_info.size()))
+        throw std::runtime_error("geometry info request index out of range");
+    if (!sim->engine) {
+        geom_info_coords(S, S.geom_info[k].ct, S.geom_info[k].geom, x, y, z);
+        return;
+    }
+    auto& c = const_cast<ffip_sim*>(sim)->geom_cache;
+    if (c[0].size() != S.geom_info.size())
+        for (auto& v : c)
+            v.assign(S.geom_info.size(), {});
+    if (c[0][k].empty() && c[1][k].empty())
+        geom_info_coords(S, S.geom_info[k].ct, S.geom_info[k].geom, c[0][k], c[1][k], c[2][k]);
+    x = c[0][k];
+    y = c[1][k];
+    z = c[2][k];
+}
+
 int ffip_sim_geom_info_count(const ffip_sim* sim, int k, int64_t* npts)
 {
     if (!npts)
         return fail(FFIP_EINVAL, "null argument");
     return guard([&] {
-        const Setup& S = need_engine(const_cast<ffip_sim*>(sim)).setup();
-        if (k < 0 || k >= int(S.geom_info.size()))
-            throw std::runtime_error("geometry info request index out of range");
         std::vector<double> x, y, z;
-        geom_info_coords(S, S.geom_info[k].ct, S.geom_info[k].geom, x, y, z);
+        geom_coords(sim, k, x, y, z);
         *npts = int64_t(x.size());
     });
 }
@@ -268,11 +304,8 @@ int ffip_sim_geom_info_count(const ffip_sim* sim, int k, int64_t* npts)
 int ffip_sim_get_geom_info(const ffip_sim* sim, int k, double* x, double* y, double* z)
 {
     return guard([&] {
-        const Setup& S = need_engine(const_cast<ffip_sim*>(sim)).setup();
-        if (k < 0 || k >= int(S.geom_info.size()))
-            throw std::runtime_error("geometry info request index out of range");
         std::vector<double> vx, vy, vz;
-        geom_info_coords(S, S.geom_info[k].ct, S.geom_info[k].geom, vx, vy, vz);
+        geom_coords(sim, k, vx, vy, vz);
         if (x)
             std::memcpy(x, vx.data(), vx.size() * 8);
         if (y)

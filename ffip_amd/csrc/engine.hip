@@ -30,7 +30,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <unordered_map>
@@ -901,12 +904,77 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// Device blocks of destroyed engines, kept for the next engine of the process:
+// repeated solves of one problem (the adjoint loop's forward and adjoint runs)
+// allocate the same sizes, and hipMalloc / hipFree both cost milliseconds and
+// synchronise the device.  Blocks are reused at their exact (rounded) size on the
+// same device; at most FFIP_DEVICE_CACHE_MB (default 4096) MiB are held, beyond
+// which released blocks are freed.  The cache lives until the process ends.
+class BlockCache {
+public:
+    static BlockCache& get()
+    {
+        static BlockCache* c = new BlockCache;  // never destroyed: no HIP calls at exit
+        return *c;
+    }
+    static size_t round(size_t bytes)
+    {
+        const size_t g = bytes >= (size_t(1) << 20) ? (size_t(1) << 20) : 256;
+        return (bytes + g - 1) / g * g;
+    }
+    void* take(size_t bytes)
+    {
+        int dev = 0;
+        HIP_OK(hipGetDevice(&dev));
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            auto it = free_.find({dev, bytes});
+            if (it != free_.end()) {
+                void* p = it->second;
+                free_.erase(it);
+                held_ -= bytes;
+                return p;
+            }
+        }
+        void* p = nullptr;
+        HIP_OK(hipMalloc(&p, bytes));
+        return p;
+    }
+    void give(void* p, size_t bytes)
+    {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || bytes > cap_) {
+            (void)hipFree(p);
+            return;
+        }
+        std::lock_guard<std::mutex> lk(m_);
+        while (held_ + bytes > cap_ && !free_.empty()) {  // drop the largest blocks first
+            auto it = std::prev(free_.end());
+            (void)hipFree(it->second);
+            held_ -= it->first.second;
+            free_.erase(it);
+        }
+        free_.emplace(std::make_pair(dev, bytes), p);
+        held_ += bytes;
+    }
+
+private:
+    BlockCache()
+    {
+        const char* e = std::getenv("FFIP_DEVICE_CACHE_MB");
+        cap_ = size_t(e ? std::max(0.0, std::atof(e)) : 4096.0) << 20;
+    }
+    std::mutex m_;
+    std::multimap<std::pair<int, size_t>, void*> free_;
+    size_t held_ = 0, cap_ = 0;
+};
+
 struct EngineImpl {
     std::unique_ptr<Setup> S;
     bool fp32 = false;
     size_t tsz = 8;
     hipStream_t stream = nullptr;
-    std::vector<void*> allocs;
+    std::vector<std::pair<void*, size_t>> allocs;  // device blocks and their rounded sizes
     double bytes = 0;
     int64_t n = 0;  // time_step
 
@@ -956,9 +1024,9 @@ struct EngineImpl {
     {
         if (count == 0)
             count = 1;
-        void* p = nullptr;
-        HIP_OK(hipMalloc(&p, count * sizeof(X)));
-        allocs.push_back(p);
+        const size_t nb = BlockCache::round(count * sizeof(X));
+        void* p = BlockCache::get().take(nb);
+        allocs.emplace_back(p, nb);
         bytes += double(count * sizeof(X));
         if (zero)
             HIP_OK(hipMemsetAsync(p, 0, count * sizeof(X), stream));
@@ -973,8 +1041,10 @@ struct EngineImpl {
         if (n > samples_cap) {
             HIP_OK(hipStreamSynchronize(stream));
             if (samples) {
-                allocs.erase(std::find(allocs.begin(), allocs.end(), static_cast<void*>(samples)));
-                HIP_OK(hipFree(samples));
+                auto it = std::find_if(allocs.begin(), allocs.end(),
+                                       [&](const std::pair<void*, size_t>& a) { return a.first == samples; });
+                BlockCache::get().give(it->first, it->second);
+                allocs.erase(it);
             }
             samples = dalloc<double>(n);
             samples_cap = n;
@@ -982,17 +1052,22 @@ struct EngineImpl {
         return samples;
     }
     template <typename X>
-    X* upload(const std::vector<X>& v)
+    X* upload(const X* v, size_t n)
     {
-        // synchronous copy: the host vector may be a temporary
-        X* p = dalloc<X>(v.size(), false);
-        if (!v.empty())
-            HIP_OK(hipMemcpy(p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+        // synchronous copy: the host array may be a temporary
+        X* p = dalloc<X>(n, false);
+        if (n)
+            HIP_OK(hipMemcpy(p, v, n * sizeof(X), hipMemcpyHostToDevice));
         else {
             X zero{};
             HIP_OK(hipMemcpy(p, &zero, sizeof(X), hipMemcpyHostToDevice));
         }
         return p;
+    }
+    template <typename X>
+    X* upload(const std::vector<X>& v)
+    {
+        return upload(v.data(), v.size());
     }
 
     void build();
@@ -1056,8 +1131,9 @@ EngineImpl::~EngineImpl()
 {
     if (stream)
         (void)hipStreamSynchronize(stream);
-    for (void* p : allocs)
-        (void)hipFree(p);
+    // the stream is idle: the blocks can serve the next engine
+    for (auto& a : allocs)
+        BlockCache::get().give(a.first, a.second);
     if (stream)
         (void)hipStreamDestroy(stream);
 }
@@ -1168,11 +1244,12 @@ void EngineImpl::build()
             Pb[h] = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
         }
         if (!M.uniform) {
-            std::vector<uint32_t> packed(size_t(L.alloc), 0u);
-            for (int c = 0; c < 3; ++c)
-                for (size_t q = 0; q < packed.size(); ++q)
-                    packed[q] |= uint32_t(M.ids[c][q]) << (8 * c);
-            ids[h] = upload(packed);
+            const size_t na = size_t(L.alloc);
+            std::unique_ptr<uint32_t[]> packed(new uint32_t[na]);
+            const uint8_t *i0 = M.ids[0].data(), *i1 = M.ids[1].data(), *i2 = M.ids[2].data();
+            for (size_t q = 0; q < na; ++q)
+                packed[q] = uint32_t(i0[q]) | uint32_t(i1[q]) << 8 | uint32_t(i2[q]) << 16;
+            ids[h] = upload(packed.get(), na);
             std::vector<MatEntry> me(M.table.eps.size());
             for (size_t e = 0; e < me.size(); ++e)
                 me[e] = {M.table.inv_eps[e], M.table.eps[e], M.table.npoles[e], M.table.pole0[e]};

@@ -5,10 +5,14 @@
 #include "setup.hpp"
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <cmath>
+#include <cstring>
 #include <limits>
 #include <memory>
 #include <stdexcept>
+#include <thread>
 
 namespace ffip_amd {
 
@@ -132,6 +136,30 @@ static Medium json2medium(const Json& j)
 }
 
 // Abstract_Susceptibility, src/medium.cpp:24-31
+// body(k) for every k in [k0, k1], spread over up to 16 host threads; the set-up
+// loops that use it write disjoint planes.
+template <class F>
+static void parallel_planes(int k0, int k1, F&& body)
+{
+    const int n = k1 - k0 + 1;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nt = std::max(1, std::min({int(hw ? hw : 1u), 16, n / 2}));
+    if (nt == 1) {
+        for (int k = k0; k <= k1; ++k)
+            body(k);
+        return;
+    }
+    std::atomic<int> next{k0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&] {
+            for (int k = next.fetch_add(1); k <= k1; k = next.fetch_add(1))
+                body(k);
+        });
+    for (auto& th : pool)
+        th.join();
+}
+
 static void abstract_sus(const Sus& s, double dt, double& c1, double& c2, double& c3)
 {
     double c0 = 2 * s.b2 + s.b1 * dt;
@@ -181,6 +209,19 @@ struct Geometry {
     virtual void bbox(double lo[3], double hi[3]) const = 0;
     virtual bool per_point() const { return false; }
     virtual AbsMedium medium_at(double, double, double) const { return med; }
+    // epsilon (half 0) or mu (half 1) at a point, with the np susceptibility
+    // amplitudes of that half written to amp: the values medium_at gives
+    // true when half_at is the same at every point, with no susceptibility,
+    // whatever the point: value then holds that epsilon (half 0) or mu (half 1)
+    virtual bool half_constant(int, double&) const { return false; }
+    virtual double half_at(double x, double y, double z, int half, double* amp, size_t np) const
+    {
+        AbsMedium m = medium_at(x, y, z);
+        const std::vector<double>& a = half == 0 ? m.e_amp : m.m_amp;
+        for (size_t p = 0; p < np; ++p)
+            amp[p] = p < a.size() ? a[p] : 0.0;
+        return half == 0 ? m.eps : m.mu;
+    }
     AbsMedium med;
 };
 
@@ -286,6 +327,31 @@ struct TwoMediumGeom : BoxGeom {
                        (x - p1[0]) / (p2[0] - p1[0]) * (dim[0] - 1)};
         int dz[3] = {dim[2], dim[1], dim[0]};
         return interpn(rho.data(), dz, q, 3);
+    }
+    bool half_constant(int half, double& value) const override
+    {
+        // m1 v + m2 (1 - v) is exactly 1 for m1 = m2 = 1 (1 - v rounds by at most
+        // 2^-54 when v < 1/2 and is exact otherwise), and exactly 0 for zero amplitudes
+        const double g1 = half == 0 ? m1.eps : m1.mu, g2 = half == 0 ? m2.eps : m2.mu;
+        if (g1 != 1 || g2 != 1)
+            return false;
+        for (const std::vector<double>* a : {half == 0 ? &m1.e_amp : &m1.m_amp, half == 0 ? &m2.e_amp : &m2.m_amp})
+            for (double x : *a)
+                if (x != 0)
+                    return false;
+        value = 1;
+        return true;
+    }
+    double half_at(double x, double y, double z, int half, double* amp, size_t np) const override
+    {
+        // the arithmetic of medium_at, without building the medium
+        double val = rho_at(x, y, z);
+        double om = 1 - val;
+        const std::vector<double>& a1 = half == 0 ? m1.e_amp : m1.m_amp;
+        const std::vector<double>& a2 = half == 0 ? m2.e_amp : m2.m_amp;
+        for (size_t p = 0; p < np; ++p)
+            amp[p] = p < a1.size() ? a1[p] * val + a2[p] * om : 0.0;
+        return half == 0 ? m1.eps * val + m2.eps * om : m1.mu * val + m2.mu * om;
     }
     AbsMedium medium_at(double x, double y, double z) const override
     {
@@ -530,39 +596,69 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
                 geom_id[g] = TB.entry(geoms[g]->med);
         bool any_dense = false;
         int dense_const_id[3] = {-1, -1, -1};
+        // owner[cc]: first geometry (list order) strictly containing each point of
+        // the box ob[cc] that covers every geometry's bounding box; points outside
+        // it belong to no geometry, so only the geometries' extent is scanned
         std::vector<std::vector<int>> owner(3);
+        int ob[3][3][2];
+        bool partial[3] = {false, false, false};  // some real points lie outside ob
         for (int cc = 0; cc < 3; ++cc) {
             const int comp = h * 3 + cc;
             const int lo0 = L.lo[comp][0], lo1 = L.lo[comp][1], lo2 = L.lo[comp][2];
             const int e0 = L.hi[comp][0] - lo0 + 1, e1 = L.hi[comp][1] - lo1 + 1, e2 = L.hi[comp][2] - lo2 + 1;
+            for (int a = 0; a < 3; ++a) {
+                ob[cc][a][0] = L.lo[comp][a];
+                ob[cc][a][1] = L.lo[comp][a] - 1;
+            }
             if (e0 <= 0 || e1 <= 0 || e2 <= 0)
                 continue;
-            std::vector<int>& own = owner[cc];
-            own.assign(size_t(e0) * e1 * e2, -1);
             int par[3] = {comp_parity(comp, 0), comp_parity(comp, 1), comp_parity(comp, 2)};
-            // first geometry in list order that strictly contains the point
+            std::vector<std::array<int, 6>> rng(geoms.size());
+            std::vector<bool> hit(geoms.size(), false);
+            int ub[3][2] = {{1 << 30, -(1 << 30)}, {1 << 30, -(1 << 30)}, {1 << 30, -(1 << 30)}};
             for (size_t g = 0; g < geoms.size(); ++g) {
                 double bl[3], bh[3];
                 geoms[g]->bbox(bl, bh);
-                int r0[3], r1[3];
                 bool empty = false;
                 for (int a = 0; a < 3; ++a) {
                     double lo_c = std::max(bl[a], double(L.P1[a]) - 2), hi_c = std::min(bh[a], double(L.P2[a]) + 2);
                     int first = int(std::floor(lo_c)), last = int(std::ceil(hi_c));
                     int lo_i = L.ix(a, std::max(first, L.P1[a] - 1)), hi_i = L.ix(a, std::min(last, L.P2[a] + 1));
-                    r0[a] = std::max(lo_i, L.lo[comp][a]);
-                    r1[a] = std::min(hi_i, L.hi[comp][a]);
-                    if (r0[a] > r1[a])
+                    rng[g][a] = std::max(lo_i, L.lo[comp][a]);
+                    rng[g][3 + a] = std::min(hi_i, L.hi[comp][a]);
+                    if (rng[g][a] > rng[g][3 + a])
                         empty = true;
                 }
                 if (empty)
                     continue;
-                for (int k = r0[2]; k <= r1[2]; ++k) {
+                hit[g] = true;
+                for (int a = 0; a < 3; ++a) {
+                    ub[a][0] = std::min(ub[a][0], rng[g][a]);
+                    ub[a][1] = std::max(ub[a][1], rng[g][3 + a]);
+                }
+            }
+            if (ub[0][0] > ub[0][1]) {  // no geometry reaches this component
+                partial[cc] = true;
+                continue;
+            }
+            for (int a = 0; a < 3; ++a) {
+                ob[cc][a][0] = ub[a][0];
+                ob[cc][a][1] = ub[a][1];
+            }
+            const int n0 = ub[0][1] - ub[0][0] + 1, n1 = ub[1][1] - ub[1][0] + 1, n2 = ub[2][1] - ub[2][0] + 1;
+            partial[cc] = int64_t(n0) * n1 * n2 < int64_t(e0) * e1 * e2;
+            std::vector<int>& own = owner[cc];
+            own.assign(size_t(n0) * n1 * n2, -1);
+            for (size_t g = 0; g < geoms.size(); ++g) {
+                if (!hit[g])
+                    continue;
+                const std::array<int, 6>& r = rng[g];
+                parallel_planes(r[2], r[5], [&](int k) {
                     double z = L.coord(2, k, par[2]);
-                    for (int j = r0[1]; j <= r1[1]; ++j) {
+                    for (int j = r[1]; j <= r[4]; ++j) {
                         double y = L.coord(1, j, par[1]);
-                        int* row = &own[(size_t(k - lo2) * e1 + (j - lo1)) * e0 - lo0];
-                        for (int i = r0[0]; i <= r1[0]; ++i) {
+                        int* row = &own[(size_t(k - ub[2][0]) * n1 + (j - ub[1][0])) * n0 - ub[0][0]];
+                        for (int i = r[0]; i <= r[3]; ++i) {
                             if (row[i] >= 0)
                                 continue;
                             double x = L.coord(0, i, par[0]);
@@ -570,24 +666,47 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
                                 row[i] = int(g);
                         }
                     }
-                }
+                });
             }
+            // index into own of lattice point (i, j, k) inside ob
+            auto oix = [&](int i, int j, int k) {
+                return (size_t(k - ub[2][0]) * n1 + (j - ub[1][0])) * n0 + (i - ub[0][0]);
+            };
             // dense region bounding box: points owned by two-medium boxes
             DenseRegion& D = HM.dense[cc];
             int bl[3] = {1 << 30, 1 << 30, 1 << 30}, bh[3] = {-1, -1, -1};
-            for (int k = 0; k < e2; ++k)
-                for (int j = 0; j < e1; ++j)
-                    for (int i = 0; i < e0; ++i) {
-                        int g = own[(size_t(k) * e1 + j) * e0 + i];
+            for (int k = ub[2][0]; k <= ub[2][1]; ++k)
+                for (int j = ub[1][0]; j <= ub[1][1]; ++j)
+                    for (int i = ub[0][0]; i <= ub[0][1]; ++i) {
+                        int g = own[oix(i, j, k)];
                         if (g >= 0 && geoms[g]->per_point()) {
-                            int q[3] = {i + lo0, j + lo1, k + lo2};
+                            int q[3] = {i, j, k};
                             for (int a = 0; a < 3; ++a) {
                                 bl[a] = std::min(bl[a], q[a]);
                                 bh[a] = std::max(bh[a], q[a]);
                             }
                         }
                     }
-            if (bh[0] >= 0) {
+            // per-point media that are provably the same everywhere (mu = 1 on the
+            // magnetic side, typically) collapse to a table entry without evaluation
+            bool fast_const = bh[0] >= 0, have_val = false;
+            double fast_val = 0;
+            for (size_t g = 0; g < geoms.size() && fast_const; ++g) {
+                double v;
+                if (!geoms[g]->per_point())
+                    continue;
+                if (!geoms[g]->half_constant(h, v) || (have_val && v != fast_val))
+                    fast_const = false;
+                fast_val = v;
+                have_val = true;
+            }
+            if (fast_const) {
+                AbsMedium m;
+                m.eps = m.mu = fast_val;
+                m.e_amp.assign(S.pool[0].size(), 0.0);
+                m.m_amp.assign(S.pool[1].size(), 0.0);
+                dense_const_id[cc] = TB.entry(m);
+            } else if (bh[0] >= 0) {
                 D.present = true;
                 for (int a = 0; a < 3; ++a) {
                     D.box[a][0] = bl[a];
@@ -605,22 +724,22 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
                 D.eps.assign(n, 1.0);
                 D.amp.assign(n * np, 0.0);
                 int d0 = bh[0] - bl[0] + 1, d1 = bh[1] - bl[1] + 1;
-                for (int k = bl[2]; k <= bh[2]; ++k)
+                parallel_planes(bl[2], bh[2], [&](int k) {
+                    std::vector<double> amp(np);
                     for (int j = bl[1]; j <= bh[1]; ++j)
                         for (int i = bl[0]; i <= bh[0]; ++i) {
-                            int g = own[(size_t(k - lo2) * e1 + (j - lo1)) * e0 + (i - lo0)];
+                            int g = own[oix(i, j, k)];
                             if (g < 0 || !geoms[g]->per_point())
                                 continue;
-                            AbsMedium m = geoms[g]->medium_at(L.coord(0, i, par[0]), L.coord(1, j, par[1]),
-                                                              L.coord(2, k, par[2]));
-                            double gi = h == 0 ? m.eps : m.mu;
-                            const std::vector<double>& amp = h == 0 ? m.e_amp : m.m_amp;
+                            double gi = geoms[g]->half_at(L.coord(0, i, par[0]), L.coord(1, j, par[1]),
+                                                          L.coord(2, k, par[2]), h, amp.data(), np);
                             int64_t q = (int64_t(k - bl[2]) * d1 + (j - bl[1])) * d0 + (i - bl[0]);
                             D.inv_eps[q] = 1 / gi;
                             D.eps[q] = gi;
                             for (size_t p = 0; p < np; ++p)
                                 D.amp[q * np + p] = std::abs(amp[p]) > 1e-4 ? amp[p] : 0.0;
                         }
+                });
                 // a region whose points all carry the same pole-free medium (for
                 // instance mu = 1 on the magnetic side) collapses to a table entry
                 bool constant = true;
@@ -629,7 +748,7 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
                 for (int k = bl[2]; k <= bh[2] && constant; ++k)
                     for (int j = bl[1]; j <= bh[1] && constant; ++j)
                         for (int i = bl[0]; i <= bh[0]; ++i) {
-                            int g = own[(size_t(k - lo2) * e1 + (j - lo1)) * e0 + (i - lo0)];
+                            int g = own[oix(i, j, k)];
                             if (g < 0 || !geoms[g]->per_point())
                                 continue;
                             int64_t q = (int64_t(k - bl[2]) * d1 + (j - bl[1])) * d0 + (i - bl[0]);
@@ -669,16 +788,23 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
                 return geom_id[g];
             return dense_const_id[cc] >= 0 ? dense_const_id[cc] : 255;
         };
-        for (int cc = 0; cc < 3 && uniform; ++cc)
+        for (int cc = 0; cc < 3 && uniform; ++cc) {
+            if (partial[cc]) {  // real points outside the box take the default medium
+                if (seen < 0)
+                    seen = bg_id;
+                if (seen != bg_id)
+                    uniform = false;
+            }
             for (int g : owner[cc]) {
+                if (!uniform)
+                    break;
                 int id = id_of(cc, g);
                 if (seen < 0)
                     seen = id;
-                if (id != seen) {
+                if (id != seen)
                     uniform = false;
-                    break;
-                }
             }
+        }
         if (uniform && seen >= 0 && HM.table.npoles[seen] != 0)
             uniform = false;
         if (uniform && seen < 0)
@@ -693,19 +819,32 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
             const int comp = h * 3 + cc;
             const int lo0 = L.lo[comp][0], lo1 = L.lo[comp][1], lo2 = L.lo[comp][2];
             const int e0 = L.hi[comp][0] - lo0 + 1, e1 = L.hi[comp][1] - lo1 + 1, e2 = L.hi[comp][2] - lo2 + 1;
-            if (!uniform)
-                HM.ids[cc].assign(L.alloc, 0);
-            if (S.keep_geom_index)
+            if (S.keep_geom_index) {
                 S.geom_index[comp] = owner[cc];
-            if (uniform || owner[cc].empty())
+                for (int a = 0; a < 3; ++a) {
+                    S.geom_box[comp][a][0] = ob[cc][a][0];
+                    S.geom_box[comp][a][1] = ob[cc][a][1];
+                }
+            }
+            if (uniform)
                 continue;
-            for (int k = 0; k < e2; ++k)
-                for (int j = 0; j < e1; ++j)
-                    for (int i = 0; i < e0; ++i) {
-                        int g = owner[cc][(size_t(k) * e1 + j) * e0 + i];
-                        int id = id_of(cc, g);
-                        HM.ids[cc][L.index(i + lo0, j + lo1, k + lo2)] = uint8_t(id);
-                    }
+            // padding cells keep id 0; real points outside ob take the default medium
+            HM.ids[cc].assign(L.alloc, 0);
+            if (e0 <= 0 || e1 <= 0 || e2 <= 0)
+                continue;
+            uint8_t* ids = HM.ids[cc].data();
+            parallel_planes(lo2, lo2 + e2 - 1, [&](int k) {
+                for (int j = lo1; j < lo1 + e1; ++j)
+                    std::memset(ids + L.index(lo0, j, k), bg_id, size_t(e0));
+            });
+            const int b0 = ob[cc][0][0], b1 = ob[cc][1][0], b2 = ob[cc][2][0];
+            const int n0 = ob[cc][0][1] - b0 + 1, n1 = ob[cc][1][1] - b1 + 1;
+            parallel_planes(b2, ob[cc][2][1], [&](int k) {
+                size_t q = size_t(k - b2) * n1 * n0;
+                for (int j = b1; j < b1 + n1; ++j)
+                    for (int i = b0; i < b0 + n0; ++i, ++q)
+                        ids[L.index(i, j, k)] = uint8_t(id_of(cc, owner[cc][q]));
+            });
         }
     }
 }
@@ -957,14 +1096,16 @@ void geom_info_coords(const Setup& S, int ct, int geom, std::vector<double>& x, 
         throw std::runtime_error("geometry info was not requested at set-up");
     const std::vector<int>& own = S.geom_index[comp];
     const Layout& L = S.L;
-    int e0 = L.hi[comp][0] - L.lo[comp][0] + 1, e1 = L.hi[comp][1] - L.lo[comp][1] + 1;
+    const int(*box)[2] = S.geom_box[comp];
+    int e0 = box[0][1] - box[0][0] + 1, e1 = box[1][1] - box[1][0] + 1;
     int par[3] = {comp_parity(comp, 0), comp_parity(comp, 1), comp_parity(comp, 2)};
+    // the box is scanned x fastest, the order of the reference's Yee_Iterator
     for (size_t q = 0; q < own.size(); ++q) {
         if (own[q] != geom)
             continue;
-        int i = int(q % e0) + L.lo[comp][0];
-        int j = int((q / e0) % e1) + L.lo[comp][1];
-        int k = int(q / (size_t(e0) * e1)) + L.lo[comp][2];
+        int i = int(q % e0) + box[0][0];
+        int j = int((q / e0) % e1) + box[1][0];
+        int k = int(q / (size_t(e0) * e1)) + box[2][0];
         // the reference stores these in std::vector<int> (src/simulation.cpp:201-211)
         x.push_back(double(int(L.coord(0, i, par[0]) * S.dx2)));
         y.push_back(double(int(L.coord(1, j, par[1]) * S.dx2)));

@@ -176,7 +176,10 @@ struct Setup {
     std::vector<std::vector<ProbeTerm>> probes;
     std::vector<FaceCopy> faces[2];
     std::vector<GeomInfoRequest> geom_info;
-    std::vector<std::vector<int>> geom_index;  // per component: geometry of each point (-1 = default)
+    // per component: geometry owning each point of the box geom_box (lattice
+    // indices, inclusive; -1 = default medium, as is every point outside the box)
+    std::vector<std::vector<int>> geom_index;
+    int geom_box[6][3][2] = {};
     std::string precision = "fp64";
     bool keep_geom_index = false;
 };

@@ -86,7 +86,8 @@ int ffip_sim_component_dims(const ffip_sim *sim, const char *component, int64_t 
 int ffip_sim_get_component(ffip_sim *sim, const char *component, double *out);
 
 /* Geometry info request k (Simulation::geometry_info_request, src/simulation.cpp:190-233):
- * coordinates of the points of a component inside geometry "geometry index". */
+ * coordinates of the points of a component inside geometry "geometry index".  Before
+ * ffip_sim_finalize_setup these run the host set-up for the call (no GPU needed). */
 int ffip_sim_geom_info_count(const ffip_sim *sim, int k, int64_t *npts);
 int ffip_sim_get_geom_info(const ffip_sim *sim, int k, double *x, double *y, double *z);
 

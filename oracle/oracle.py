@@ -726,6 +726,21 @@ class OracleSim:
                 grp["c"] = np.array([coefs[i] for i in grp["poles"]]).reshape(-1, 3)
 
     # -- src/fields.cpp:183-207
+    def geometry_info(self, comp, geom):
+        """Simulation::geometry_info_request (src/simulation.cpp:190-233): the
+        points of component `comp` whose first containing geometry is `geom`, in
+        Yee_Iterator order (x fastest), as coordinate * dx/2 stored in the
+        reference's std::vector<int> (truncated towards zero) and written as doubles."""
+        if comp not in self.geom_index:
+            return np.zeros(0), np.zeros(0), np.zeros(0)
+        p1, dims, gidx = self.geom_index[comp]
+        k, j, i = np.nonzero(gidx == geom)  # C order: z slowest, x fastest
+        out = []
+        for a, n in ((0, i), (1, j), (2, k)):
+            c = (p1[a] + 2 * n).astype(np.float64) * self.dx2
+            out.append(np.trunc(c))
+        return tuple(out)
+
     def _add_dipole_g1(self, pos, ct, amp, start, freq, cutoff):
         amp = -amp
         base = self.grid.base_point(pos, ct)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from ffip_amd import workloads as W
-from ffip_amd._lib import plan
+from ffip_amd._lib import geometry_info, plan
 from oracle import oracle as O
 
 EH = O.EH_ORDER
@@ -206,3 +206,40 @@ def test_probe_weights(pair):
             X = _coords(o, [idx])
             assert ci == int(_comp_of(X)[0])
             assert w == wo
+
+
+def _geom_info_case(resolution):
+    """Overlapping box, sphere and two-medium box; requests for every geometry
+    on E and H components."""
+    cfg = W.base((12, 11, 10), 2)
+    cfg["resolution"] = resolution
+    cfg["geometry"] = [{"type": "box", "center": [1.0, -0.5, 0.25], "size": [5.0, 4.0, 3.5],
+                        "medium": W.medium(3.0)},
+                       {"type": "sphere", "center": [-1.3, 1.2, 0.4], "radius": 2.6, "medium": W.medium(2.0)},
+                       {"type": "two medium box", "center": [0.0, 0.0, -1.0], "size": [6.0, 5.0, 3.0],
+                        "dimension": [4, 3, 2], "density dataset": "rho", "medium1": W.medium(4.0),
+                        "medium2": W.medium(1.0)}]
+    reqs = []
+    for g in range(3):
+        for comp in ("Ex", "Ey", "Ez", "Hx", "Hz"):
+            reqs.append({"field component": comp, "geometry index": g,
+                         "output group": "geometry info request %d %s" % (g, comp)})
+    cfg["geometry info request"] = reqs
+    ds = {"rho": np.random.default_rng(4).random(24)}
+    return cfg, ds
+
+
+@pytest.mark.parametrize("resolution", [1, 2])
+def test_geometry_info_matches_oracle(resolution):
+    cfg, ds = _geom_info_case(resolution)
+    got = geometry_info(cfg, ds)
+    o = O.OracleSim(cfg, ds, trace_steps=1, strict_pool=False)
+    total = 0
+    for r, (x, y, z) in zip(cfg["geometry info request"], got):
+        wx, wy, wz = o.geometry_info(r["field component"], r["geometry index"])
+        assert np.array_equal(x, wx) and np.array_equal(y, wy) and np.array_equal(z, wz), r
+        total += x.size
+    assert total > 0
+    # points owned by an earlier geometry are not reported for a later one
+    counts = [g[0].size for g in got]
+    assert counts[0] > 0 and counts[5] > 0 and counts[10] > 0
