@@ -1,6 +1,8 @@
 // extern "C" boundary of libffip_hip.so (declared in include/ffip_hip.h).
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <memory>
@@ -111,11 +113,21 @@ int ffip_sim_finalize_setup(ffip_sim* sim)
     if (sim->engine)
         return fail(FFIP_ESTATE, "already finalized");
     return guard([&] {
+        // FFIP_TIMING=1 reports the host and device halves of the set-up on stderr
+        static const bool report = std::getenv("FFIP_TIMING") != nullptr;
+        auto t0 = std::chrono::steady_clock::now();
         auto S = std::make_unique<Setup>();
         build_setup(sim->config, sim->datasets, sim->probes, *S);
+        auto t1 = std::chrono::steady_clock::now();
         sim->engine = std::make_unique<Engine>(std::move(S));
         if (sim->trace_cap)
             sim->engine->set_trace_capacity(sim->trace_cap);
+        if (report) {
+            auto t2 = std::chrono::steady_clock::now();
+            std::fprintf(stderr, "[ffip] set-up: host %.1f ms, device %.1f ms\n",
+                         std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                         std::chrono::duration<double, std::milli>(t2 - t1).count());
+        }
     });
 }
 
@@ -464,6 +476,19 @@ std::string plan_json(const Setup& S)
         o.key("uniform");
         o.inum(M.uniform ? 1 : 0);
         o.s += ",";
+        o.key("default_id");
+        o.inum(M.bg_id);
+        o.s += ",";
+        o.key("mbox");  // lattice box holding every non-default point
+        o.s += "[";
+        for (int a = 0; a < 3; ++a) {
+            o.s += a ? ",[" : "[";
+            o.inum(M.mbox[a][0]);
+            o.s += ",";
+            o.inum(M.mbox[a][1]);
+            o.s += "]";
+        }
+        o.s += "],";
         o.key("table");
         o.s += "[";
         for (size_t e = 0; e < M.table.eps.size(); ++e) {

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -101,6 +102,8 @@ struct HalfArgs {
     // materials
     double u_inv, u_eps;
     const uint32_t* ids;  // per cell: id of component 0 | 1 << 8 | 2 << 16
+    int mbox[3][2];       // cells outside this box hold bg_word (not read from ids)
+    uint32_t bg_word;
     const MatEntry* mats;
     int nmats;
     const double* coef;  // [npoles][3] = (c1, c2, sigma*c3) of the table's poles
@@ -520,11 +523,21 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
     constexpr bool POL = MAT && NPM > 0 && CP;
     const int seg0 = (j - A.dlo[1]) * A.gx + tx;
     const int segplane = A.seg_ny * A.gx;
+    // columns and planes outside the box of non-default material use the
+    // default medium's word without loading it
+    const bool mcol = i >= A.mbox[0][0] && i <= A.mbox[0][1] && j >= A.mbox[1][0] && j <= A.mbox[1][1];
+    auto mat_word = [&](int kk) -> uint32_t {
+        if (!MAT)
+            return 0u;
+        if (!mcol || kk < A.mbox[2][0] || kk > A.mbox[2][1])
+            return A.bg_word;
+        return ldb(A.ids, uint32_t(X.base + sz * kk) * 4u);
+    };
     uint32_t idn[U], sbn[U][3];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
         const int kk = min(kb + q, ke);
-        idn[q] = MAT ? ldb(A.ids, uint32_t(X.base + sz * kk) * 4u) : 0u;
+        idn[q] = mat_word(kk);
 #pragma unroll
         for (int C = 0; C < 3; ++C)
             sbn[q][C] = POL ? A.segbase[C][(kk - A.dlo[2]) * segplane + seg0] : 0u;
@@ -568,7 +581,7 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
 #pragma unroll
         for (int q = 0; q < U; ++q) {
             const int kk = min(kc + U + q, ke);
-            idn[q] = MAT ? ldb(A.ids, uint32_t(X.base + sz * kk) * 4u) : 0u;
+            idn[q] = mat_word(kk);
 #pragma unroll
             for (int C = 0; C < 3; ++C)
                 sbn[q][C] = POL ? A.segbase[C][(kk - A.dlo[2]) * segplane + seg0] : 0u;
@@ -650,6 +663,42 @@ __global__ void __launch_bounds__(256) k_dft_sparse(const SparseDftArgs<T> A)
                 re[(f0 + q) * stride] = a[q] + ph[2 * (f0 + q)] * v;
                 im[(f0 + q) * stride] = b[q] + ph[2 * (f0 + q) + 1] * v;
             }
+    }
+}
+
+// Single-frequency units (the adjoint loop's monitors): PT points per thread,
+// strided by the grid, with every load of a thread issued before any update.
+template <typename T, int PT>
+__global__ void __launch_bounds__(256) k_dft_sparse1(const SparseDftArgs<T> A)
+{
+    const DftDev& U = A.units[blockIdx.y];
+    const int n = int(U.npts);
+    const int stride = int(gridDim.x * blockDim.x);
+    const int t0 = int(blockIdx.x * blockDim.x + threadIdx.x);
+    const T* u = A.u[U.comp];
+    const double c = A.ph[2 * U.mf0], s = A.ph[2 * U.mf0 + 1];
+    double v[PT], a[PT], b[PT];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const int t = t0 + p * stride;
+        if (t < n) {
+            const int ii = t % U.dims[0];
+            const int r = t / U.dims[0];
+            const int jj = r % U.dims[1];
+            const int kk = r / U.dims[1];
+            const int o = (U.box[0][0] + ii) + A.pitch * (U.box[1][0] + jj) + A.plane * (U.box[2][0] + kk);
+            v[p] = double(u[o]);
+            a[p] = U.re[t];
+            b[p] = U.im[t];
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+        const int t = t0 + p * stride;
+        if (t < n) {
+            U.re[t] = a[p] + c * v[p];
+            U.im[t] = b[p] + s * v[p];
+        }
     }
 }
 
@@ -1119,7 +1168,7 @@ struct EngineImpl {
         }
     }
     bool pcompact[2] = {false, false};  // polarization layout of each half
-    int unroll = 4, unroll_mat = 1;  // z planes whose loads are issued together
+    int unroll = 2, unroll_mat = 2;  // z planes whose loads are issued together (pole-free / pole kernels)
     void ensure_phasors();
     template <typename T>
     void step_once(hipEvent_t* ev, bool timed_e, bool timed_h);
@@ -1196,15 +1245,26 @@ PolLayoutPlan plan_polarization(const Setup& s, int h)
 
 void EngineImpl::build()
 {
+    // FFIP_TIMING=1 reports where the device set-up spends its time on stderr
+    static const bool report = std::getenv("FFIP_TIMING") != nullptr;
+    auto last = std::chrono::steady_clock::now();
+    auto mark = [&](const char* what) {
+        if (!report)
+            return;
+        auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[ffip]   %s %.2f ms\n", what, std::chrono::duration<double, std::milli>(now - last).count());
+        last = now;
+    };
     Setup& s = *S;
     const Layout& L = s.L;
     fp32 = s.precision == "fp32";
     tsz = fp32 ? 4 : 8;
-    // measured on MI355X (256^3 dispersive and 200x200x100 design workloads, FP32
-    // and FP64): 8-plane z segments and two planes per load batch; kernels whose
-    // points carry more than four poles fall back to one plane per batch
+    // measured on MI355X (256^3 dispersive, 512^3 voxel-mix and 200x200x100
+    // design workloads, FP32 and FP64): 8-plane z segments; kernels with poles
+    // issue two planes per load batch (one when a point carries more than two
+    // poles), the others two in FP32 and one in FP64
     lz = 8;
-    unroll = 2;
+    unroll = fp32 ? 2 : 1;
     unroll_mat = 2;
     // tuning overrides: z segment per workgroup, planes per load batch
     if (const char* e = std::getenv("FFIP_LZ"))
@@ -1243,6 +1303,7 @@ void EngineImpl::build()
             Pa[h] = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
             Pb[h] = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
         }
+        mark("fields and polarization");
         if (!M.uniform) {
             const size_t na = size_t(L.alloc);
             std::unique_ptr<uint32_t[]> packed(new uint32_t[na]);
@@ -1281,6 +1342,7 @@ void EngineImpl::build()
                 d_c[h][c] = upload(cc);
             }
         }
+        mark("materials");
         // CPML: E points use even-parity tables, H points odd-parity ones
         const int par = h == 0 ? 0 : 1;
         for (int a = 0; a < 3; ++a) {
@@ -1300,6 +1362,7 @@ void EngineImpl::build()
             D.psi1 = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
             D.psi2 = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
         }
+        mark("cpml");
         // DFT units
         std::vector<double> fl;
         for (auto& U : s.units[h]) {
@@ -1342,6 +1405,7 @@ void EngineImpl::build()
         MF[h] = int(fl.size());
         freqs_dev[h] = upload(fl);
         ph[h] = dalloc<double>(size_t(std::max(1, MF[h])) * 2 * PH_CHUNK);
+        mark("dft units");
         // sources: group dipoles per target point (insertion order kept)
         std::vector<std::pair<int64_t, int>> key;  // (index, comp)
         std::vector<std::vector<int>> members;
@@ -1422,6 +1486,7 @@ void EngineImpl::build()
             SR.munit = upload(munit);
             SR.mofs = upload(mofs);
         }
+        mark("sources");
         // faces
         for (auto& F : s.faces[h]) {
             FaceDev d{};
@@ -1441,6 +1506,7 @@ void EngineImpl::build()
         if (faces[h].size() > size_t(MAXF))
             throw std::runtime_error("too many boundary faces");
     }
+    mark("faces");
     // algorithmic traffic per half-step launch
     for (int h = 0; h < 2; ++h) {
         const HalfMaterial& M = s.mat[h];
@@ -1449,17 +1515,36 @@ void EngineImpl::build()
             nreal += double(L.count(h * 3 + c));
             nsrc += double(L.count((1 - h) * 3 + c));
         }
-        if (!M.uniform)
+        double mcells = 0;  // cells whose material word is read (inside M.mbox)
+        if (!M.uniform) {
+            int lo[3], hi[3];
+            mcells = 1;
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = M.mbox[a][0];
+                hi[a] = M.mbox[a][1];
+                int dlo = 1 << 30, dhi = -(1 << 30);
+                for (int c = 0; c < 3; ++c) {
+                    dlo = std::min(dlo, L.lo[h * 3 + c][a]);
+                    dhi = std::max(dhi, L.hi[h * 3 + c][a]);
+                }
+                mcells *= std::max(0, std::min(hi[a], dhi) - std::max(lo[a], dlo) + 1);
+            }
+            const int bg_np = M.table.npoles[M.bg_id];
             for (int c = 0; c < 3; ++c) {
                 const int comp = h * 3 + c;
-                for (int k = L.lo[comp][2]; k <= L.hi[comp][2]; ++k)
-                    for (int j = L.lo[comp][1]; j <= L.hi[comp][1]; ++j)
-                        for (int i = L.lo[comp][0]; i <= L.hi[comp][0]; ++i) {
+                // points outside the box carry the default medium's poles
+                double inside = 0;
+                for (int k = std::max(lo[2], L.lo[comp][2]); k <= std::min(hi[2], L.hi[comp][2]); ++k)
+                    for (int j = std::max(lo[1], L.lo[comp][1]); j <= std::min(hi[1], L.hi[comp][1]); ++j)
+                        for (int i = std::max(lo[0], L.lo[comp][0]); i <= std::min(hi[0], L.hi[comp][0]); ++i) {
                             int id = M.ids[c][L.index(i, j, k)];
                             int np = id == 255 ? M.dense[c].npoles : M.table.npoles[id];
                             ade += 3.0 * np;  // read P, read P-, write P+
+                            inside += 1;
                         }
+                ade += 3.0 * bg_np * (double(L.count(comp)) - inside);
             }
+        }
         const int par = h == 0 ? 0 : 1;
         for (int c = 0; c < 3; ++c) {
             const int comp = h * 3 + c;
@@ -1480,13 +1565,14 @@ void EngineImpl::build()
                 dftb += double(U.npts) * U.nf * 32.0;  // re, im: read + write, FP64
         double* t = traffic[h];
         t[0] = (2 * nreal + nsrc) * double(tsz);
-        t[1] = M.uniform ? 0.0 : 4.0 * nreal / 3.0;  // one packed 32-bit id word per cell
+        t[1] = 4.0 * mcells;  // one packed 32-bit id word per cell of the material box
         t[2] = ade * double(tsz);
         t[3] = psi * double(tsz);
         t[4] = dftb;
         t[5] = t[0] + t[1] + t[2] + t[3] + t[4];
         t[6] = nreal;
     }
+    mark("traffic model");
     // probes
     nprobes = int(s.probes.size());
     if (nprobes) {
@@ -1506,7 +1592,9 @@ void EngineImpl::build()
         pr_idx = upload(idx);
         pr_w = upload(w);
     }
+    mark("probes");
     HIP_OK(hipStreamSynchronize(stream));
+    mark("copies and fills");
 }
 
 template <typename T>
@@ -1543,6 +1631,11 @@ HalfArgs<T> EngineImpl::half_args(int h)
     A.u_eps = M.u_eps;
     if (!M.uniform) {
         A.ids = ids[h];
+        for (int a = 0; a < 3; ++a) {
+            A.mbox[a][0] = M.mbox[a][0];
+            A.mbox[a][1] = M.mbox[a][1];
+        }
+        A.bg_word = uint32_t(M.bg_id) * 0x010101u;
         A.mats = mats[h];
         A.nmats = nmats[h];
         A.coef = p_coef[h];
@@ -1626,10 +1719,11 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
     if (A.dhi[0] < A.dlo[0] || A.dhi[1] < A.dlo[1] || A.dhi[2] < A.dlo[2])
         return;
     dim3 block(BX, BY, 1);
-    dim3 grid(A.nblocks, 1, 1);
-    const bool mat = !S->mat[h].uniform;
-    const int U = mat ? unroll_mat : unroll;
+    const HalfMaterial& M = S->mat[h];
+    const bool mat = !M.uniform;
+    const int U = mat && S_slots[h] > 0 ? unroll_mat : unroll;
     const size_t lds = mat ? size_t(A.nmats + A.npoles) * 3 * sizeof(double) : 0;
+    dim3 grid(A.nblocks, 1, 1);
     if (e0)
         HIP_OK(hipEventRecord(e0, stream));
     if (h == 0)
@@ -1676,14 +1770,21 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
         D.pitch = int(S->L.pitch);
         D.plane = int(S->L.plane);
         int64_t most = 0;
+        bool single = true;
         for (auto& U : units[h])
             if (!U.fused && U.npts > 0) {
                 D.units[D.nunits++] = U;
                 most = std::max(most, U.npts);
+                single = single && U.nf == 1;
             }
         D.ph = A.ph;
         if (D.nunits > 0) {
-            k_dft_sparse<T><<<dim3(unsigned((most + 255) / 256), D.nunits), 256, 0, stream>>>(D);
+            constexpr int PT = 4;
+            if (single)
+                k_dft_sparse1<T, PT>
+                    <<<dim3(unsigned((most + 256 * PT - 1) / (256 * PT)), D.nunits), 256, 0, stream>>>(D);
+            else
+                k_dft_sparse<T><<<dim3(unsigned((most + 255) / 256), D.nunits), 256, 0, stream>>>(D);
             HIP_OK(hipGetLastError());
         }
     }

@@ -778,6 +778,18 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
         for (int cc = 0; cc < 3; ++cc)
             if (HM.dense[cc].present)
                 HM.max_poles = std::max(HM.max_poles, HM.dense[cc].npoles);
+        // points outside every owner box take the default medium
+        HM.bg_id = bg_id;
+        for (int a = 0; a < 3; ++a) {
+            HM.mbox[a][0] = 1 << 30;
+            HM.mbox[a][1] = -(1 << 30);
+        }
+        for (int cc = 0; cc < 3; ++cc)
+            if (!owner[cc].empty())
+                for (int a = 0; a < 3; ++a) {
+                    HM.mbox[a][0] = std::min(HM.mbox[a][0], ob[cc][a][0]);
+                    HM.mbox[a][1] = std::max(HM.mbox[a][1], ob[cc][a][1]);
+                }
         // ids per point; uniform when every point has the same pole-free medium
         bool uniform = !any_dense;
         int seen = -1;

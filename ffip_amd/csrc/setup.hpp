@@ -109,6 +109,10 @@ struct HalfMaterial {
     MatTable table;
     DenseRegion dense[3];
     int max_poles = 0;
+    // every point whose id is not the default medium's (bg_id) lies in this box
+    // of lattice indices (inclusive; empty when lo > hi)
+    int bg_id = 0;
+    int mbox[3][2] = {{0, -1}, {0, -1}, {0, -1}};
 };
 
 struct PmlAxis {  // per axis; [parity]

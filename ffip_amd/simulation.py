@@ -10,6 +10,7 @@ ignored: one GPU holds the whole grid.
 """
 from __future__ import annotations
 
+import json
 from copy import copy, deepcopy
 from math import pi
 
@@ -258,6 +259,7 @@ class Simulation:
         self.flux_boxes = []
         self.geom_info_requests = []
         self.engine = None
+        self._geom_key, self._geom_cache = None, {}
         for p in self.pmls:
             if p.sigma_max is None:
                 p.sigma_max = 0.8 * (1 + p.order) * resolution
@@ -324,14 +326,26 @@ class Simulation:
             config = self.get_json(stop_condition=stop_condition, decomposition=decomposition)
             store = _DatasetStore()
             self.dump(store)
+            if self.engine is not None:
+                # the previous run's results have been read; releasing its device
+                # memory first lets the new engine reuse the blocks
+                self.engine.close()
+                self.engine = None
             self.engine = LibSim(config, store.data)
             self.engine.run(config["stop condition"])
+            key = json.dumps(config, sort_keys=True)
+            if key != self._geom_key:
+                self._geom_key, self._geom_cache = key, {}
         groups = {}
         for k in range(self.engine.dft_count()):
             d = self.engine.dft(k)
             groups[d["group"]] = d
         for k, req in enumerate(self.geom_info_requests):
-            x, y, z = self.engine.geom_info(k)
+            # which points a geometry owns depends on the document only (shapes and
+            # grid, not the datasets), so an unchanged document reuses them
+            if k not in self._geom_cache:
+                self._geom_cache[k] = self.engine.geom_info(k)
+            x, y, z = self._geom_cache[k]
             groups[req.gname] = {"x": x, "y": y, "z": z}
         for item in self.dft_fields:
             item.read(groups)

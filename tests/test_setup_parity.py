@@ -243,3 +243,24 @@ def test_geometry_info_matches_oracle(resolution):
     # points owned by an earlier geometry are not reported for a later one
     counts = [g[0].size for g in got]
     assert counts[0] > 0 and counts[5] > 0 and counts[10] > 0
+
+
+def test_material_box_holds_every_geometry_point(pair):
+    """The half-step kernels read material words only inside the material box
+    and take the default medium elsewhere: every point a geometry owns (oracle
+    restatement of the first-containing-geometry rule) must lie inside it."""
+    name, cfg, p, o = pair
+    for half in (0, 1):
+        M = p["materials"][half]
+        if M["uniform"]:
+            continue
+        box = np.array(M["mbox"])
+        for n in EH[3 * half:3 * half + 3]:
+            if n not in o.geom_index:
+                continue
+            p1, dims, gidx = o.geom_index[n]
+            k, j, i = np.nonzero(gidx >= 0)
+            X = np.stack([p1[0] + 2 * i, p1[1] + 2 * j, p1[2] + 2 * k], axis=1)
+            q = _ix(p, X)
+            for a in range(3):
+                assert np.all(q[:, a] >= box[a, 0]) and np.all(q[:, a] <= box[a, 1]), (name, n, a)
