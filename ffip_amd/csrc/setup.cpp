@@ -649,12 +649,17 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
             partial[cc] = int64_t(n0) * n1 * n2 < int64_t(e0) * e1 * e2;
             std::vector<int>& own = owner[cc];
             own.assign(size_t(n0) * n1 * n2, -1);
-            for (size_t g = 0; g < geoms.size(); ++g) {
-                if (!hit[g])
-                    continue;
-                const std::array<int, 6>& r = rng[g];
-                parallel_planes(r[2], r[5], [&](int k) {
-                    double z = L.coord(2, k, par[2]);
+            // geometries touching each plane, in list order, so that planes can be
+            // filled independently with the first-containing-geometry rule
+            std::vector<std::vector<int>> on_plane(static_cast<size_t>(n2));
+            for (size_t g = 0; g < geoms.size(); ++g)
+                if (hit[g])
+                    for (int k = rng[g][2]; k <= rng[g][5]; ++k)
+                        on_plane[size_t(k - ub[2][0])].push_back(int(g));
+            parallel_planes(ub[2][0], ub[2][1], [&](int k) {
+                const double z = L.coord(2, k, par[2]);
+                for (int g : on_plane[size_t(k - ub[2][0])]) {
+                    const std::array<int, 6>& r = rng[g];
                     for (int j = r[1]; j <= r[4]; ++j) {
                         double y = L.coord(1, j, par[1]);
                         int* row = &own[(size_t(k - ub[2][0]) * n1 + (j - ub[1][0])) * n0 - ub[0][0]];
@@ -663,11 +668,11 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
                                 continue;
                             double x = L.coord(0, i, par[0]);
                             if (geoms[g]->inside(x, y, z))
-                                row[i] = int(g);
+                                row[i] = g;
                         }
                     }
-                });
-            }
+                }
+            });
             // index into own of lattice point (i, j, k) inside ob
             auto oix = [&](int i, int j, int k) {
                 return (size_t(k - ub[2][0]) * n1 + (j - ub[1][0])) * n0 + (i - ub[0][0]);
