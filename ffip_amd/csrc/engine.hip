@@ -98,6 +98,9 @@ struct HalfArgs {
     int lo[3][3], hi[3][3];
     int dlo[3], dhi[3];
     int gx, gy, gz, nblocks, lz;
+    // tiles of this launch: tile t = tx + gx * (ty + gy * tz), in the order of the list
+    const uint32_t* tiles;
+    int ntiles;
     double inv_dx, dt;
     // materials
     double u_inv, u_eps;
@@ -267,7 +270,7 @@ __device__ __forceinline__ void load_fields(const HalfArgs<T>& A, int o, PlaneIn
 // word of the cell and (compact layout) sb[c] the first P entry of its row
 // segment, both fetched one batch earlier.  With the compact layout it must be
 // called by every live lane of the wave (offsets come from ballots over the row).
-template <typename T, bool MAT, int NPM, bool CP>
+template <typename T, bool MAT, int NPM, bool CP, bool DENSE>
 __device__ __forceinline__ void load_sparse(const HalfArgs<T>& A, const MatEntry* smat, const ColCtx& X,
                                             const ZCtx& Z, uint32_t id, const uint32_t sb[3], const bool act[3],
                                             PlaneIn<T, NPM, CP>& L)
@@ -297,7 +300,7 @@ __device__ __forceinline__ void load_sparse(const HalfArgs<T>& A, const MatEntry
             const int mid = int((id >> (8 * C)) & 255u);
             int np = 0;
             if (act[C]) {
-                if (mid != 255) {
+                if (!DENSE || mid != 255) {
                     np = smat[mid].np;
                 } else {
                     np = A.d_np[C];
@@ -353,7 +356,7 @@ __device__ __forceinline__ double cpml_apply(void* arr, int q, T psi, double b, 
 
 // Update component C of one point from its preloaded plane data: CPML, ADE
 // poles (src/structure.cpp:56-85, in E form), DFT accumulation, store.
-template <typename T, bool MAT, int NPM, bool CP, int C>
+template <typename T, bool MAT, int NPM, bool CP, bool DENSE, int C>
 __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry* smat, const double* spc,
                                             const ColCtx& X, const ZCtx& Z, double d1, double d2, uint32_t id,
                                             const PlaneIn<T, NPM, CP>& L)
@@ -385,7 +388,7 @@ __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry
     } else {
         mid = int((id >> (8 * C)) & 255u);
         double inv, dP = 0;
-        if (mid != 255) {
+        if (!DENSE || mid != 255) {
             const MatEntry m = smat[mid];
             inv = m.inv;
 #pragma unroll
@@ -436,7 +439,8 @@ __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry
         double v = double(stored);
         if (U.is_db) {
             const double eps = !MAT ? A.u_eps
-                                    : (mid != 255 ? smat[mid].eps : A.d_eps[C][dense_off(A, C, X.i, X.j, Z.k)]);
+                                    : ((!DENSE || mid != 255) ? smat[mid].eps
+                                                              : A.d_eps[C][dense_off(A, C, X.i, X.j, Z.k)]);
             v = eps * double(stored) + sumP;
         }
         const int64_t off = (X.i - U.box[0][0]) +
@@ -455,7 +459,9 @@ __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry
 // Loads for U consecutive planes are issued before any of them is computed so
 // that each thread keeps several planes of requests in flight; the two
 // components differentiated along z are carried between planes in registers.
-template <typename T, bool IS_E, bool MAT, int NPM, bool CP, int U>
+// PML = false: the launch's tiles lie clear of every CPML slab (no auxiliary
+// code or registers); DENSE = false: no point of this half has a per-point medium.
+template <typename T, bool IS_E, bool MAT, int NPM, bool CP, int U, bool PML, bool DENSE>
 __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
 {
     // material table and pole coefficients, staged in LDS (sized at launch:
@@ -475,13 +481,14 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
     // XCD-aware order: consecutive dispatches land on the 8 XCDs in turn, so the
     // tiles one XCD runs are made contiguous in space (shared halos stay in its L2)
     const int b = int(blockIdx.x);
-    const int t = (b & 7) * (A.nblocks >> 3) + (b >> 3);
+    const int slot = (b & 7) * (A.nblocks >> 3) + (b >> 3);
+    if (slot >= A.ntiles)
+        return;
+    const int t = int(A.tiles[slot]);
     const int tx = t % A.gx;
     const int tr = t / A.gx;
     const int ty = tr % A.gy;
     const int tz = tr / A.gy;
-    if (tz >= A.gz)
-        return;
     const int i = A.dlo[0] + tx * BX + int(threadIdx.x);
     const int j = A.dlo[1] + ty * BY + int(threadIdx.y);
     const int kb = A.dlo[2] + tz * A.lz;
@@ -497,8 +504,8 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
 #pragma unroll
     for (int C = 0; C < 3; ++C)
         X.in[C] = i >= A.lo[C][0] && i <= A.hi[C][0] && j >= A.lo[C][1] && j <= A.hi[C][1];
-    X.sx = A.pml[0].W ? A.pml[0].smap[i] : -1;
-    X.sy = A.pml[1].W ? A.pml[1].smap[j] : -1;
+    X.sx = PML && A.pml[0].W ? A.pml[0].smap[i] : -1;
+    X.sy = PML && A.pml[1].W ? A.pml[1].smap[j] : -1;
     X.bx = X.cx = X.by = X.cy = 0;
     X.ikx = X.iky = 1;
     if (X.sx >= 0) {
@@ -562,7 +569,7 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
             Z[q].sz = -1;
             Z[q].bz = Z[q].cz = 0;
             Z[q].ikz = 1;
-            if (k <= ke && A.pml[2].W) {
+            if (PML && k <= ke && A.pml[2].W) {
                 Z[q].sz = A.pml[2].smap[k];
                 if (Z[q].sz >= 0) {
                     Z[q].bz = A.pml[2].b[k];
@@ -574,7 +581,7 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
             for (int C = 0; C < 3; ++C)
                 act[q][C] = k <= ke && X.in[C] && k >= A.lo[C][2] && k <= A.hi[C][2];
             if (k <= ke)
-                load_sparse<T, MAT, NPM, CP>(A, smat, X, Z[q], idc[q], sbc[q], act[q], L[q]);
+                load_sparse<T, MAT, NPM, CP, DENSE>(A, smat, X, Z[q], idc[q], sbc[q], act[q], L[q]);
         }
         // then the loads every thread makes: next batch's material words and
         // segment bases, fields
@@ -600,17 +607,17 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
             if (act[q][0]) {
                 const double d1 = (double(P.v2b) - w2) * inv_dx;
                 const double d2 = (IS_E ? (g1 - a1) : (a1 - g1)) * inv_dx;
-                update_comp<T, MAT, NPM, CP, 0>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
+                update_comp<T, MAT, NPM, CP, DENSE, 0>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
             if (act[q][1]) {
                 const double d1 = (IS_E ? (g0 - a0) : (a0 - g0)) * inv_dx;
                 const double d2 = (double(P.v2a) - w2) * inv_dx;
-                update_comp<T, MAT, NPM, CP, 1>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
+                update_comp<T, MAT, NPM, CP, DENSE, 1>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
             if (act[q][2]) {
                 const double d1 = (double(P.v1a) - (IS_E ? a1 : g1)) * inv_dx;
                 const double d2 = (double(P.v0b) - (IS_E ? a0 : g0)) * inv_dx;
-                update_comp<T, MAT, NPM, CP, 2>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
+                update_comp<T, MAT, NPM, CP, DENSE, 2>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
             c0 = P.n0;
             c1 = P.n1;
@@ -1126,48 +1133,90 @@ struct EngineImpl {
     void launch_half(int h, hipEvent_t e0, hipEvent_t e1);
     // npm: most poles at any point of the half (register arrays are sized by it);
     // cp: compact polarization layout; U: planes whose loads are issued together
-    template <typename T, bool IS_E, bool CP>
+    // Kernel choice.  Electric material kernels with up to three poles are
+    // compiled with and without the CPML and per-point-medium paths; the other
+    // material kernels (magnetic media, more poles) keep both paths, which are
+    // correct on any tile.
+    template <typename T, bool IS_E, bool CP, bool PML, bool DENSE>
     void launch_mat(int npm, int U, size_t lds, dim3 grid, dim3 block, const HalfArgs<T>& A)
     {
-#define FFIP_K(NP, UU) k_half<T, IS_E, true, NP, CP, UU><<<grid, block, lds, stream>>>(A)
-        // two planes per batch only while the pole registers leave occupancy 3
-        if (U >= 2 && npm <= 2) {
+#define FFIP_K(NP, UU, P, D) k_half<T, IS_E, true, NP, CP, UU, P, D><<<grid, block, lds, stream>>>(A)
+        // two planes per batch only while the pole registers allow (three poles
+        // fit without the CPML paths)
+        if (U >= 2 && npm <= (PML ? 2 : 3)) {
             if (npm == 1)
-                FFIP_K(1, 2);
-            else
-                FFIP_K(2, 2);
+                FFIP_K(1, 2, PML, DENSE);
+            else if (npm == 2)
+                FFIP_K(2, 2, PML, DENSE);
+            else if constexpr (!PML)
+                FFIP_K(3, 2, PML, DENSE);
             return;
         }
         switch (npm) {
-        case 1: FFIP_K(1, 1); break;
-        case 2: FFIP_K(2, 1); break;
-        case 3: FFIP_K(3, 1); break;
-        case 4: FFIP_K(4, 1); break;
-        default: FFIP_K(MAXS, 1); break;
+        case 1: FFIP_K(1, 1, PML, DENSE); break;
+        case 2: FFIP_K(2, 1, PML, DENSE); break;
+        case 3: FFIP_K(3, 1, PML, DENSE); break;
+        case 4: FFIP_K(4, 1, true, true); break;
+        default: FFIP_K(MAXS, 1, true, true); break;
         }
 #undef FFIP_K
     }
-    template <typename T, bool IS_E>
-    void launch_k(bool mat, int npm, bool cp, int U, size_t lds, dim3 grid, dim3 block, const HalfArgs<T>& A)
+    template <typename T, bool IS_E, bool PML, bool DENSE>
+    void launch_flags(bool mat, int npm, bool cp, int U, size_t lds, dim3 grid, dim3 block, const HalfArgs<T>& A)
     {
         if (!mat) {
             switch (U) {
-            case 1: k_half<T, IS_E, false, 0, false, 1><<<grid, block, 0, stream>>>(A); break;
-            case 2: k_half<T, IS_E, false, 0, false, 2><<<grid, block, 0, stream>>>(A); break;
-            default: k_half<T, IS_E, false, 0, false, 4><<<grid, block, 0, stream>>>(A); break;
+            case 1: k_half<T, IS_E, false, 0, false, 1, PML, false><<<grid, block, 0, stream>>>(A); break;
+            case 2: k_half<T, IS_E, false, 0, false, 2, PML, false><<<grid, block, 0, stream>>>(A); break;
+            default: k_half<T, IS_E, false, 0, false, 4, PML, false><<<grid, block, 0, stream>>>(A); break;
             }
-        } else if (npm == 0) {
-            if (U >= 2)
-                k_half<T, IS_E, true, 0, false, 2><<<grid, block, lds, stream>>>(A);
+            return;
+        }
+        if constexpr (IS_E || (PML && DENSE)) {
+            if (npm == 0) {
+                if (U >= 2)
+                    k_half<T, IS_E, true, 0, false, 2, PML, DENSE><<<grid, block, lds, stream>>>(A);
+                else
+                    k_half<T, IS_E, true, 0, false, 1, PML, DENSE><<<grid, block, lds, stream>>>(A);
+            } else if (cp) {
+                launch_mat<T, IS_E, true, PML, DENSE>(npm, U, lds, grid, block, A);
+            } else {
+                launch_mat<T, IS_E, false, PML, DENSE>(npm, U, lds, grid, block, A);
+            }
+        }
+    }
+    template <typename T, bool IS_E>
+    void launch_k(bool mat, int npm, bool cp, int U, bool pml, bool dense, size_t lds, dim3 grid, dim3 block,
+                  const HalfArgs<T>& A)
+    {
+        if constexpr (!IS_E) {
+            if (mat) {  // magnetic material kernels: general form only
+                launch_flags<T, IS_E, true, true>(mat, npm, cp, U, lds, grid, block, A);
+                return;
+            }
+        }
+        if (pml) {
+            if (dense)
+                launch_flags<T, IS_E, true, true>(mat, npm, cp, U, lds, grid, block, A);
             else
-                k_half<T, IS_E, true, 0, false, 1><<<grid, block, lds, stream>>>(A);
-        } else if (cp) {
-            launch_mat<T, IS_E, true>(npm, U, lds, grid, block, A);
+                launch_flags<T, IS_E, true, false>(mat, npm, cp, U, lds, grid, block, A);
         } else {
-            launch_mat<T, IS_E, false>(npm, U, lds, grid, block, A);
+            if (dense)
+                launch_flags<T, IS_E, false, true>(mat, npm, cp, U, lds, grid, block, A);
+            else
+                launch_flags<T, IS_E, false, false>(mat, npm, cp, U, lds, grid, block, A);
         }
     }
     bool pcompact[2] = {false, false};  // polarization layout of each half
+    // tiles of the half step: list a (launched without the CPML paths when
+    // pml_a is false) and list b (with them)
+    uint32_t* tiles_a[2] = {};
+    uint32_t* tiles_b[2] = {};
+    int ntiles_a[2] = {0, 0}, ntiles_b[2] = {0, 0};
+    bool pml_a[2] = {true, true};
+    bool dense[2] = {false, false};  // some point of the half has a per-point medium
+    template <typename T>
+    void build_tiles();
     int unroll = 2, unroll_mat = 2;  // z planes whose loads are issued together (pole-free / pole kernels)
     void ensure_phasors();
     template <typename T>
@@ -1593,6 +1642,11 @@ void EngineImpl::build()
         pr_w = upload(w);
     }
     mark("probes");
+    if (fp32)
+        build_tiles<float>();
+    else
+        build_tiles<double>();
+    mark("tile lists");
     HIP_OK(hipStreamSynchronize(stream));
     mark("copies and fills");
 }
@@ -1713,6 +1767,63 @@ void EngineImpl::ensure_phasors()
 }
 
 template <typename T>
+void EngineImpl::build_tiles()
+{
+    const Setup& s = *S;
+    for (int h = 0; h < 2; ++h) {
+        for (int c = 0; c < 3; ++c)
+            dense[h] = dense[h] || s.mat[h].dense[c].present;
+        HalfArgs<T> A = half_args<T>(h);
+        if (A.dhi[0] < A.dlo[0] || A.dhi[1] < A.dlo[1] || A.dhi[2] < A.dlo[2])
+            continue;
+        // per axis: does tile q reach a CPML slab (of this half's parity)?
+        const int par = h == 0 ? 0 : 1, step[3] = {BX, BY, A.lz}, n[3] = {A.gx, A.gy, A.gz};
+        std::vector<char> touch[3];
+        for (int a = 0; a < 3; ++a) {
+            touch[a].assign(size_t(n[a]), 0);
+            const PmlAxis& P = s.pml[a];
+            if (P.W[par] == 0)
+                continue;
+            for (int q = 0; q < n[a]; ++q)
+                for (int i = A.dlo[a] + q * step[a]; i <= std::min(A.dlo[a] + (q + 1) * step[a] - 1, A.dhi[a]); ++i)
+                    if (P.smap[par][i] >= 0)
+                        touch[a][q] = 1;
+        }
+        std::vector<uint32_t> in, sh;
+        for (int tz = 0; tz < A.gz; ++tz)
+            for (int ty = 0; ty < A.gy; ++ty)
+                for (int tx = 0; tx < A.gx; ++tx) {
+                    const uint32_t t = uint32_t(tx + A.gx * (ty + A.gy * tz));
+                    (touch[0][tx] || touch[1][ty] || touch[2][tz] ? sh : in).push_back(t);
+                }
+        // Two launches pay only where the CPML-free kernel gains occupancy (the
+        // material kernels) and each launch still fills the GPU several times
+        // over (measured on MI355X: 256^3 and 512^3 gain 13 %, a 200x200x100
+        // grid of 2275 tiles loses 10 %); otherwise one launch over every tile.
+        // FFIP_CPML_SPLIT=1 / 0 forces / forbids the split wherever both lists are non-empty
+        const char* force = std::getenv("FFIP_CPML_SPLIT");
+        bool split = !s.mat[h].uniform && in.size() + sh.size() >= 4096;
+        if (force)
+            split = std::atoi(force) != 0;
+        split = split && !sh.empty() && !in.empty();
+        if (split) {
+            pml_a[h] = false;
+        } else if (sh.empty()) {
+            pml_a[h] = false;  // no CPML reaches this half's tiles
+        } else {
+            pml_a[h] = true;
+            in.insert(in.end(), sh.begin(), sh.end());
+            std::sort(in.begin(), in.end());
+            sh.clear();
+        }
+        ntiles_a[h] = int(in.size());
+        ntiles_b[h] = int(sh.size());
+        tiles_a[h] = upload(in);
+        tiles_b[h] = upload(sh);
+    }
+}
+
+template <typename T>
 void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
 {
     HalfArgs<T> A = half_args<T>(h);
@@ -1723,14 +1834,25 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
     const bool mat = !M.uniform;
     const int U = mat && S_slots[h] > 0 ? unroll_mat : unroll;
     const size_t lds = mat ? size_t(A.nmats + A.npoles) * 3 * sizeof(double) : 0;
-    dim3 grid(A.nblocks, 1, 1);
+    // tiles clear of the CPML without its paths (when split), then the others
+    auto go = [&](const uint32_t* list, int count, bool pml) {
+        if (count == 0)
+            return;
+        HalfArgs<T> X = A;
+        X.tiles = list;
+        X.ntiles = count;
+        X.nblocks = ((count + 7) / 8) * 8;
+        dim3 grid(X.nblocks, 1, 1);
+        if (h == 0)
+            launch_k<T, true>(mat, S_slots[h], pcompact[h], U, pml, dense[h], lds, grid, block, X);
+        else
+            launch_k<T, false>(mat, S_slots[h], pcompact[h], U, pml, dense[h], lds, grid, block, X);
+        HIP_OK(hipGetLastError());
+    };
     if (e0)
         HIP_OK(hipEventRecord(e0, stream));
-    if (h == 0)
-        launch_k<T, true>(mat, S_slots[h], pcompact[h], U, lds, grid, block, A);
-    else
-        launch_k<T, false>(mat, S_slots[h], pcompact[h], U, lds, grid, block, A);
-    HIP_OK(hipGetLastError());
+    go(tiles_a[h], ntiles_a[h], pml_a[h]);
+    go(tiles_b[h], ntiles_b[h], true);
     if (e1)
         HIP_OK(hipEventRecord(e1, stream));
     if (S_slots[h] > 0)

@@ -94,6 +94,16 @@ def test_setup_cases_fp64(name):
     _compare(cfg, ds, probes, steps=150, label=name)
 
 
+@pytest.mark.parametrize("name", ["spheres", "design", "voxels", "odd_grid"])
+def test_cpml_split_kernels_match_oracle(name, monkeypatch):
+    """Tiles clear of every CPML slab run kernels compiled without the CPML
+    paths (and, without per-point media, without that path too); large grids use
+    them by default.  Forced here on small cases, against the oracle."""
+    monkeypatch.setenv("FFIP_CPML_SPLIT", "1")
+    cfg, ds, probes = CASES[name]
+    _compare(cfg, ds, probes, steps=150, label=name + "-split")
+
+
 def test_dispersive_spheres_fp64_and_fp32():
     cfg = W.dispersive_spheres(n=48, nspheres=6, steps=600, pml_cells=6, nfreq=4, precision="fp64")
     _compare(cfg, label="spheres48-fp64")
