@@ -978,10 +978,8 @@ public:
         const size_t g = bytes >= (size_t(1) << 20) ? (size_t(1) << 20) : 256;
         return (bytes + g - 1) / g * g;
     }
-    void* take(size_t bytes)
+    void* take(size_t bytes, int dev)
     {
-        int dev = 0;
-        HIP_OK(hipGetDevice(&dev));
         {
             std::lock_guard<std::mutex> lk(m_);
             auto it = free_.find({dev, bytes});
@@ -996,10 +994,10 @@ public:
         HIP_OK(hipMalloc(&p, bytes));
         return p;
     }
-    void give(void* p, size_t bytes)
+    // p was allocated on device dev, which must be current
+    void give(void* p, size_t bytes, int dev)
     {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess || bytes > cap_) {
+        if (bytes > cap_) {
             (void)hipFree(p);
             return;
         }
@@ -1031,6 +1029,7 @@ struct EngineImpl {
     size_t tsz = 8;
     hipStream_t stream = nullptr;
     std::vector<std::pair<void*, size_t>> allocs;  // device blocks and their rounded sizes
+    int device = 0;                                 // current device when the engine was built
     double bytes = 0;
     int64_t n = 0;  // time_step
 
@@ -1081,7 +1080,7 @@ struct EngineImpl {
         if (count == 0)
             count = 1;
         const size_t nb = BlockCache::round(count * sizeof(X));
-        void* p = BlockCache::get().take(nb);
+        void* p = BlockCache::get().take(nb, device);
         allocs.emplace_back(p, nb);
         bytes += double(count * sizeof(X));
         if (zero)
@@ -1099,7 +1098,7 @@ struct EngineImpl {
             if (samples) {
                 auto it = std::find_if(allocs.begin(), allocs.end(),
                                        [&](const std::pair<void*, size_t>& a) { return a.first == samples; });
-                BlockCache::get().give(it->first, it->second);
+                BlockCache::get().give(it->first, it->second, device);
                 allocs.erase(it);
             }
             samples = dalloc<double>(n);
@@ -1229,9 +1228,15 @@ EngineImpl::~EngineImpl()
 {
     if (stream)
         (void)hipStreamSynchronize(stream);
-    // the stream is idle: the blocks can serve the next engine
+    // the stream is idle: the blocks can serve the next engine on this device
+    int cur = 0;
+    const bool switched = hipGetDevice(&cur) == hipSuccess && cur != device;
+    if (switched)
+        (void)hipSetDevice(device);
     for (auto& a : allocs)
-        BlockCache::get().give(a.first, a.second);
+        BlockCache::get().give(a.first, a.second, device);
+    if (switched)
+        (void)hipSetDevice(cur);
     if (stream)
         (void)hipStreamDestroy(stream);
 }
@@ -1326,6 +1331,7 @@ void EngineImpl::build()
     if (L.alloc * int64_t(tsz) >= (int64_t(1) << 32))
         throw std::runtime_error("grid too large: one field array must stay below 4 GiB (" +
                                  std::to_string(L.alloc) + " cells)");
+    HIP_OK(hipGetDevice(&device));
     HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (int c = 0; c < 6; ++c) {
         field[c] = fp32 ? (void*)dalloc<float>(L.alloc) : (void*)dalloc<double>(L.alloc);

@@ -12,6 +12,7 @@
 #include <limits>
 #include <memory>
 #include <stdexcept>
+#include <system_error>
 #include <thread>
 
 namespace ffip_amd {
@@ -150,12 +151,19 @@ static void parallel_planes(int k0, int k1, F&& body)
         return;
     }
     std::atomic<int> next{k0};
+    auto work = [&] {
+        for (int k = next.fetch_add(1); k <= k1; k = next.fetch_add(1))
+            body(k);
+    };
     std::vector<std::thread> pool;
-    for (int t = 0; t < nt; ++t)
-        pool.emplace_back([&] {
-            for (int k = next.fetch_add(1); k <= k1; k = next.fetch_add(1))
-                body(k);
-        });
+    for (int t = 1; t < nt; ++t) {
+        try {
+            pool.emplace_back(work);
+        } catch (const std::system_error&) {
+            break;  // no more threads: the calling thread takes the rest
+        }
+    }
+    work();
     for (auto& th : pool)
         th.join();
 }

@@ -104,6 +104,24 @@ def test_cpml_split_kernels_match_oracle(name, monkeypatch):
     _compare(cfg, ds, probes, steps=150, label=name + "-split")
 
 
+@pytest.mark.parametrize("split", ["0", "1"])
+def test_dispersive_default_medium(split, monkeypatch):
+    """A dispersive default medium with pole-free boxes in it: points outside the
+    box of non-default media take the default's material word without reading
+    it, and those points carry poles (compact or full polarization storage)."""
+    monkeypatch.setenv("FFIP_CPML_SPLIT", split)
+    cfg = W.base((18, 16, 14), 3)
+    cfg["default material"] = W.medium(1.5, e_sus=[W.lorentz(0.08, 0.01, 0.6)])
+    cfg["geometry"] = [{"type": "box", "center": [1.0, 0.5, 0.0], "size": [4.0, 3.0, 3.0], "medium": W.medium(3.0)},
+                       {"type": "sphere", "center": [-3.0, -2.0, 1.0], "radius": 2.2,
+                        "medium": W.medium(2.0, e_sus=[W.drude(0.15, 0.02, 0.5)])}]
+    cfg["sources"] = [{"type": "volume source", "center": [0.0, 0.0, -3.0], "size": [6.0, 6.0, 0.0],
+                       "amplitude": 1.0, "field component": "Ex", "function": W.gaussian1(0.1)}]
+    cfg["fields output"] = [W.dft((0, 0, 2.5), (12, 10, 0), [0.07, 0.1], "Ex", "a")]
+    cfg["stop condition"] = {"type": "time", "time": 100.0}
+    _compare(cfg, steps=200, label="dispersive-default-split%s" % split)
+
+
 def test_dispersive_spheres_fp64_and_fp32():
     cfg = W.dispersive_spheres(n=48, nspheres=6, steps=600, pml_cells=6, nfreq=4, precision="fp64")
     _compare(cfg, label="spheres48-fp64")
