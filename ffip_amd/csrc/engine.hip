@@ -642,7 +642,11 @@ __global__ void __launch_bounds__(256) k_dft_sparse(const SparseDftArgs<T> A)
 {
     const DftDev& U = A.units[blockIdx.y];
     const int t = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);  // unit sizes are checked < 2^31
-    if (t >= int(U.npts))
+    // frequencies in chunks of FC, one chunk per grid row (blockIdx.z): all sums
+    // of a chunk are read before any is written
+    constexpr int FC = 4;
+    const int f0 = int(blockIdx.z) * FC;
+    if (t >= int(U.npts) || f0 >= U.nf)
         return;
     const int ii = t % U.dims[0];
     const int r = t / U.dims[0];
@@ -654,9 +658,7 @@ __global__ void __launch_bounds__(256) k_dft_sparse(const SparseDftArgs<T> A)
     double* __restrict__ re = U.re + t;
     double* __restrict__ im = U.im + t;
     const int64_t stride = U.npts;
-    // frequencies in chunks: all sums of a chunk are read before any is written
-    constexpr int FC = 8;
-    for (int f0 = 0; f0 < U.nf; f0 += FC) {
+    {
         double a[FC], b[FC];
 #pragma unroll
         for (int q = 0; q < FC; ++q)
@@ -1908,11 +1910,16 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
         D.ph = A.ph;
         if (D.nunits > 0) {
             constexpr int PT = 4;
-            if (single)
+            if (single) {
                 k_dft_sparse1<T, PT>
                     <<<dim3(unsigned((most + 256 * PT - 1) / (256 * PT)), D.nunits), 256, 0, stream>>>(D);
-            else
-                k_dft_sparse<T><<<dim3(unsigned((most + 255) / 256), D.nunits), 256, 0, stream>>>(D);
+            } else {
+                int mf = 1;
+                for (int u = 0; u < D.nunits; ++u)
+                    mf = std::max(mf, D.units[u].nf);
+                k_dft_sparse<T>
+                    <<<dim3(unsigned((most + 255) / 256), D.nunits, unsigned((mf + 3) / 4)), 256, 0, stream>>>(D);
+            }
             HIP_OK(hipGetLastError());
         }
     }
