@@ -200,10 +200,13 @@ class Surrogate:
                 L.adam(self.t, lr)
         return tot / max(cnt, 1)
 
-    def fit(self, rhos, targets, epochs=60, batch=8, lr=2e-3, seed=0, log=None):
+    def fit(self, rhos, targets, epochs=60, batch=8, lr=2e-3, seed=0, log=None, augment=False):
         """Mean-squared error on targets [N, 21, 21, nout] divided by self.scale
-        (set here from the targets and kept for later fine-tuning)."""
+        (set here from the targets and kept for later fine-tuning); augment adds
+        the mirror images (mirror_augment)."""
         torch = _torch()
+        if augment:
+            rhos, targets = mirror_augment(rhos, targets)
         self.scale = float(np.mean(np.abs(targets))) or 1.0
         tgt = torch.as_tensor(np.asarray(targets, dtype=np.float32) / self.scale, device="cuda")
         rng = np.random.default_rng(seed)
@@ -214,10 +217,12 @@ class Surrogate:
                 log("epoch %d: train mse %.4e" % (ep, hist[-1]))
         return hist
 
-    def fine_tune(self, rhos, targets, steps=20, batch=4, lr=1e-3, seed=0):
+    def fine_tune(self, rhos, targets, steps=20, batch=4, lr=1e-3, seed=0, augment=False):
         """A few Adam steps on a small set of (density, target) pairs (online
         updates from the solves of an optimisation run)."""
         torch = _torch()
+        if augment:
+            rhos, targets = mirror_augment(rhos, targets)
         tgt = torch.as_tensor(np.asarray(targets, dtype=np.float32) / self.scale, device="cuda")
         rng = np.random.default_rng(seed + self.t)
         n = len(rhos)
@@ -265,6 +270,18 @@ def patch_fields(mon):
     return np.moveaxis(v, 0, -1)  # [y, x, f]
 
 
+def mirror_augment(rhos, targets):
+    """The inverse-design workload is mirror-symmetric in x and in y (grid, CPML,
+    source sheet, design nodes and patch are all centred), so the density flipped
+    along x and/or y has the patch intensities flipped the same way: four
+    training pairs per solve.  rhos [N, z, y, x], targets [N, y, x, f]."""
+    rhos = np.asarray(rhos)
+    targets = np.asarray(targets)
+    r = [rhos, rhos[..., ::-1], rhos[..., ::-1, :], rhos[..., ::-1, ::-1]]
+    t = [targets, targets[:, :, ::-1, :], targets[:, ::-1, :, :], targets[:, ::-1, ::-1, :]]
+    return np.ascontiguousarray(np.concatenate(r)), np.ascontiguousarray(np.concatenate(t))
+
+
 def to_intensity(fields):
     """[..., f] complex patch fields -> [..., f] intensities |E|^2 (the network's targets)."""
     return (np.abs(fields) ** 2).astype(np.float32)
@@ -287,7 +304,24 @@ def generate(seeds, steps=3000, precision="fp32", log=None):
     return np.stack(rhos), np.stack(fields)
 
 
-def ascent(problem, rho0, iters, surrogate=None, monitor=None, alpha0=0.05, inner=1, tune_steps=20, log=None):
+class FixedChoice:
+    """Stands in for the network in `ascent` (an ablation): always picks the
+    candidate step length of index `k` (3 = the longest of 0.5, 1, 2, 4 x alpha)."""
+
+    def __init__(self, k=3):
+        self.k = int(k)
+
+    def values(self, trials):
+        v = np.zeros(len(trials))
+        v[self.k] = 1.0
+        return v
+
+    def fine_tune(self, *args, **kwargs):
+        pass
+
+
+def ascent(problem, rho0, iters, surrogate=None, monitor=None, alpha0=0.05, inner=1, tune_steps=20, log=None,
+           augment=False):
     """Normalised projected-gradient ascent of F on [0, 1] with full-solve
     acceptance.  Without a surrogate, step lengths are found by backtracking
     with forward solves.  With one, `inner` surrogate-guided steps (step length
@@ -308,7 +342,7 @@ def ascent(problem, rho0, iters, surrogate=None, monitor=None, alpha0=0.05, inne
         if surrogate is not None and monitor is not None:
             seen_r.append(np.asarray(c, dtype=np.float32))
             seen_t.append(to_intensity(patch_fields(monitor)))
-            surrogate.fine_tune(np.stack(seen_r[-16:]), np.stack(seen_t[-16:]), steps=tune_steps)
+            surrogate.fine_tune(np.stack(seen_r[-16:]), np.stack(seen_t[-16:]), steps=tune_steps, augment=augment)
         return F
 
     F = forward(rho0)
@@ -322,12 +356,14 @@ def ascent(problem, rho0, iters, surrogate=None, monitor=None, alpha0=0.05, inne
         g = np.where(((rho >= 1) & (g > 0)) | ((rho <= 0) & (g < 0)), 0.0, g)
         d = g / (np.abs(g).max() + 1e-300)
         cand = None
+        picked = None
         if surrogate is not None:
             c, step, first = rho, d, None
             for k in range(inner):
                 lens = alpha * np.array([0.5, 1.0, 2.0, 4.0])
                 trials = np.stack([np.clip(c + a * step, 0, 1) for a in lens])
                 best = int(np.argmax(surrogate.values(trials.astype(np.float32))))
+                picked = [0.5, 1.0, 2.0, 4.0][best] if picked is None else picked
                 c = trials[best]
                 first = lens[best] if first is None else first
                 if k + 1 < inner:
@@ -353,7 +389,8 @@ def ascent(problem, rho0, iters, surrogate=None, monitor=None, alpha0=0.05, inne
         rho = cand
         hist.append((count[0], F))
         if log:
-            log("iter %d: F = %.6e, full solves %d" % (it, F, count[0]))
+            log("iter %d: F = %.6e, full solves %d%s" % (it, F, count[0],
+                                                       "" if picked is None else ", picked %g x alpha" % picked))
     return rho, F, hist, count[0]
 
 
@@ -366,9 +403,10 @@ def main(argv=None):
 
     `data` solves the forward problem for the blurred random densities of seeds
     1000.. (FP32 storage) and saves the patch fields; `study` trains the network
-    on all but the last 64 designs, reports its held-out error, then runs the
-    plain and the surrogate-assisted gradient ascent from the same start and
-    counts full solves."""
+    on all but the last 64 designs (with their mirror images), reports its
+    held-out error, then runs the plain, the surrogate-assisted and, as an
+    ablation, the same policy with the longest candidate step always chosen (no
+    network), from the same start, and counts full solves."""
     import argparse
     import json
     import sys
@@ -384,6 +422,7 @@ def main(argv=None):
     ap.add_argument("--inner", type=int, default=1)
     ap.add_argument("--tune-steps", type=int, default=20)
     ap.add_argument("--heldout", type=int, default=64)
+    ap.add_argument("--no-augment", action="store_true", help="train without the mirror images")
     a = ap.parse_args(argv)
 
     def log(msg):
@@ -403,7 +442,7 @@ def main(argv=None):
     ntr = len(seeds) - a.heldout
     net = Surrogate(seed=0)
     t0 = time.time()
-    hist = net.fit(rhos[:ntr], to_intensity(fields[:ntr]), epochs=a.epochs, log=log)
+    hist = net.fit(rhos[:ntr], to_intensity(fields[:ntr]), epochs=a.epochs, log=log, augment=not a.no_augment)
     t_train = time.time() - t0
     pred = np.concatenate([net.predict(rhos[q:q + 8]) for q in range(ntr, len(seeds), 8)])
     true = to_intensity(fields[ntr:])
@@ -418,11 +457,12 @@ def main(argv=None):
 
     rho0 = blurred_density(shape, seed=2000, radius=3, sigma=1.5)
     runs = {}
-    for name, sg in (("plain", None), ("surrogate", net)):
+    for name, sg in (("plain", None), ("surrogate", net), ("longest-step", FixedChoice(3))):
         p, mon = design_problem(a.steps, "fp32")
         t0 = time.time()
         rho, F, h, total = ascent(p, rho0, a.iters, surrogate=sg, monitor=mon, inner=a.inner,
-                                  tune_steps=a.tune_steps, log=lambda m: log(name + " " + m))
+                                  tune_steps=a.tune_steps, log=lambda m: log(name + " " + m),
+                                  augment=not a.no_augment)
         runs[name] = {"final_F": F, "history": h, "total_solves": total, "seconds": time.time() - t0}
     target = runs["plain"]["final_F"]
     for name, r in runs.items():
@@ -433,10 +473,11 @@ def main(argv=None):
     res = {"train_mse": hist, "train_seconds": t_train, "heldout_intensity_rel_l2": field_err,
            "heldout_objective_median_rel_err": F_err, "heldout_objective_corr": corr,
            "designs": int(len(seeds)), "train_designs": int(ntr), "iters": a.iters, "inner": a.inner,
-           "tune_steps": a.tune_steps, "runs": runs}
-    if runs["surrogate"]["solves_to_plain_final"]:
-        res["solve_ratio_at_plain_final"] = runs["plain"]["solves_to_plain_final"] / runs["surrogate"][
-            "solves_to_plain_final"]
+           "tune_steps": a.tune_steps, "augment": not a.no_augment, "runs": runs}
+    for name in ("surrogate", "longest-step"):
+        if runs[name]["solves_to_plain_final"]:
+            res["solve_ratio_at_plain_final_" + name] = (runs["plain"]["solves_to_plain_final"] /
+                                                        runs[name]["solves_to_plain_final"])
     with open(a.out, "w") as f:
         json.dump(res, f, indent=1, default=float)
     log(json.dumps({k: v for k, v in res.items() if k not in ("train_mse", "runs")}, default=float))

@@ -175,3 +175,32 @@ def test_training_reduces_loss_and_input_gradient():
     fd = (net.values(rp[None], fidx=1)[0] - net.values(rm[None], fidx=1)[0]) / (2 * h)
     # activations are bf16, so this checks the wiring (sign, magnitude), not digits
     assert abs(fd - g[q]) < 0.3 * abs(g[q]) + 1e-6, (fd, g[q])
+
+
+def test_mirror_augment_flips_densities_and_targets():
+    rng = np.random.default_rng(2)
+    r = rng.random((2, 3, 5, 7))
+    t = rng.random((2, 4, 6, 3))
+    R, T = SG.mirror_augment(r, t)
+    assert R.shape == (8, 3, 5, 7) and T.shape == (8, 4, 6, 3)
+    assert np.array_equal(R[2], r[0][..., ::-1]) and np.array_equal(T[2], t[0][:, ::-1, :])   # x
+    assert np.array_equal(R[4], r[0][:, ::-1, :]) and np.array_equal(T[4], t[0][::-1, :, :])  # y
+    assert np.array_equal(R[7], r[1][:, ::-1, ::-1]) and np.array_equal(T[7], t[1][::-1, ::-1, :])
+
+
+@pytest.mark.gpu
+def test_design_workload_is_mirror_symmetric():
+    """The premise of mirror_augment: the solver's patch intensities for a
+    density flipped along x (y) are the original ones flipped along x (y)."""
+    from ffip_amd.workloads import blurred_density
+    p, mon = SG.design_problem(steps=1200, precision="fp64")
+    rho = blurred_density(p.shape, seed=1001, radius=3, sigma=1.5)
+    out = []
+    for r in (rho, rho[..., ::-1], rho[:, ::-1, :]):
+        p.objective(np.ascontiguousarray(r))
+        out.append(SG.to_intensity(SG.patch_fields(mon)).astype(np.float64))
+    base = out[0]
+    assert np.linalg.norm(base) > 0
+    for got, want in ((out[1], base[:, ::-1, :]), (out[2], base[::-1, :, :])):
+        err = np.linalg.norm(got - want) / np.linalg.norm(want)
+        assert err < 1e-9, err
