@@ -533,12 +533,26 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
     // columns and planes outside the box of non-default material use the
     // default medium's word without loading it
     const bool mcol = i >= A.mbox[0][0] && i <= A.mbox[0][1] && j >= A.mbox[1][0] && j <= A.mbox[1][1];
+    // FP64: every lane loads (the id array spans the whole lattice); cells
+    // outside the box read element 0, one cached word, and take the default word
+    // held in a register.  FP32: the choice between the argument word and the
+    // id array compiles to one flat load through the chosen pointer.  Measured
+    // on MI355X, same box, E half step: FP64 256^3 dispersive 460 us (global
+    // load) vs 486 us (flat), 200x200x100 design 94.7 vs 99.0 us; FP32 the other
+    // way round (256^3 241 vs 255 us, 512^3 voxel mix 2956 vs 3002 us).
+    const uint32_t bg_word = __builtin_amdgcn_readfirstlane(A.bg_word);
     auto mat_word = [&](int kk) -> uint32_t {
         if (!MAT)
             return 0u;
-        if (!mcol || kk < A.mbox[2][0] || kk > A.mbox[2][1])
-            return A.bg_word;
-        return ldb(A.ids, uint32_t(X.base + sz * kk) * 4u);
+        const bool in = mcol && kk >= A.mbox[2][0] && kk <= A.mbox[2][1];
+        if constexpr (sizeof(T) == 4) {
+            if (!in)
+                return A.bg_word;
+            return ldb(A.ids, uint32_t(X.base + sz * kk) * 4u);
+        } else {
+            const uint32_t w = ldb(A.ids, in ? uint32_t(X.base + sz * kk) * 4u : 0u);
+            return in ? w : bg_word;
+        }
     };
     uint32_t idn[U], sbn[U][3];
 #pragma unroll
@@ -1219,6 +1233,7 @@ struct EngineImpl {
     template <typename T>
     void build_tiles();
     int unroll = 2, unroll_mat = 2;  // z planes whose loads are issued together (pole-free / pole kernels)
+    int unroll_shell = 0;            // pole kernels on CPML tiles of a split half step (0: unroll_mat)
     void ensure_phasors();
     template <typename T>
     void step_once(hipEvent_t* ev, bool timed_e, bool timed_h);
@@ -1318,10 +1333,12 @@ void EngineImpl::build()
     // measured on MI355X (256^3 dispersive, 512^3 voxel-mix and 200x200x100
     // design workloads, FP32 and FP64): 8-plane z segments; kernels with poles
     // issue two planes per load batch (one when a point carries more than two
-    // poles), the others two in FP32 and one in FP64
+    // poles, and in FP64 on the CPML tiles of a split half step), the others
+    // two in FP32 and one in FP64
     lz = 8;
     unroll = fp32 ? 2 : 1;
     unroll_mat = 2;
+    unroll_shell = fp32 ? 0 : 1;
     // tuning overrides: z segment per workgroup, planes per load batch
     if (const char* e = std::getenv("FFIP_LZ"))
         lz = std::max(1, std::atoi(e));
@@ -1329,6 +1346,8 @@ void EngineImpl::build()
         unroll = std::atoi(e);
     if (const char* e = std::getenv("FFIP_UNROLL_MAT"))
         unroll_mat = std::atoi(e);
+    if (const char* e = std::getenv("FFIP_UNROLL_SHELL"))
+        unroll_shell = std::atoi(e);
     // the half-step kernel addresses every per-cell array with a 32-bit byte offset
     if (L.alloc * int64_t(tsz) >= (int64_t(1) << 32))
         throw std::runtime_error("grid too large: one field array must stay below 4 GiB (" +
@@ -1846,15 +1865,16 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
     auto go = [&](const uint32_t* list, int count, bool pml) {
         if (count == 0)
             return;
+        const int u = pml && !pml_a[h] && unroll_shell > 0 && mat && S_slots[h] > 0 ? unroll_shell : U;
         HalfArgs<T> X = A;
         X.tiles = list;
         X.ntiles = count;
         X.nblocks = ((count + 7) / 8) * 8;
         dim3 grid(X.nblocks, 1, 1);
         if (h == 0)
-            launch_k<T, true>(mat, S_slots[h], pcompact[h], U, pml, dense[h], lds, grid, block, X);
+            launch_k<T, true>(mat, S_slots[h], pcompact[h], u, pml, dense[h], lds, grid, block, X);
         else
-            launch_k<T, false>(mat, S_slots[h], pcompact[h], U, pml, dense[h], lds, grid, block, X);
+            launch_k<T, false>(mat, S_slots[h], pcompact[h], u, pml, dense[h], lds, grid, block, X);
         HIP_OK(hipGetLastError());
     };
     if (e0)

@@ -7,7 +7,10 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -1142,6 +1145,16 @@ void geom_info_coords(const Setup& S, int ct, int geom, std::vector<double>& x, 
 void build_setup(const Json& cfg, const DatasetMap& ds, const std::vector<ProbeSpec>& probes, Setup& S)
 {
     // Simulation::init, src/simulation.cpp:127-188
+    // FFIP_TIMING=1 reports where the host set-up spends its time on stderr
+    static const bool report = std::getenv("FFIP_TIMING") != nullptr;
+    auto tick = std::chrono::steady_clock::now();
+    auto mark = [&](const char* what) {
+        if (!report)
+            return;
+        auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[ffip]   host %s %.2f ms\n", what, std::chrono::duration<double, std::milli>(now - tick).count());
+        tick = now;
+    };
     S.courant = cfg.num_or("courant number", 0.5);
     S.resolution = cfg.at("resolution").num();
     if (!(S.resolution > 0))
@@ -1192,6 +1205,7 @@ void build_setup(const Json& cfg, const DatasetMap& ds, const std::vector<ProbeS
         S.bg = json2medium(*dm);
     add_to_pool(S.pool, S.bg);
     set_pmls(cfg.find("PML"), S);
+    mark("layout and cpml");
 
     // geometry, src/simulation.cpp:359-400
     std::vector<std::unique_ptr<Geometry>> geoms;
@@ -1212,6 +1226,7 @@ void build_setup(const Json& cfg, const DatasetMap& ds, const std::vector<ProbeS
         for (auto& g : gj->elems())
             geoms.push_back(build_geometry(g, S.pool, ds, S.dx2));
     }
+    mark("geometry");
     if (const Json* gi = cfg.find("geometry info request"))
         for (auto& r : gi->elems()) {
             S.geom_info.push_back({r.at("output group").str(), ctype_from_name(r.at("field component").str()),
@@ -1219,10 +1234,12 @@ void build_setup(const Json& cfg, const DatasetMap& ds, const std::vector<ProbeS
             S.keep_geom_index = true;
         }
     set_materials(geoms, S);
+    mark("materials");
 
     if (const Json* sj = cfg.find("sources"))
         for (auto& s : sj->elems())
             set_source(S, s, ds);
+    mark("sources");
     if (const Json* fo = cfg.find("fields output"))
         for (auto& f : fo->elems())
             register_dft(S, f);
@@ -1234,6 +1251,7 @@ void build_setup(const Json& cfg, const DatasetMap& ds, const std::vector<ProbeS
     for (auto& p : probes)
         S.probes.push_back(probe_terms(S, p.pos, p.ct));
     set_faces(S);
+    mark("dft units, probes and faces");
 }
 
 }  // namespace ffip_amd
