@@ -649,18 +649,24 @@ struct SparseDftArgs {
     int nunits;
     DftDev units[MAXU];
     const double* ph;
+    int npb, nch;  // k_dft_sparse: 256-point blocks and frequency chunks per unit
 };
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_dft_sparse(const SparseDftArgs<T> A)
 {
     const DftDev& U = A.units[blockIdx.y];
-    const int t = int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x);  // unit sizes are checked < 2^31
-    // frequencies in chunks of FC, one chunk per grid row (blockIdx.z): all sums
-    // of a chunk are read before any is written
+    // frequencies in chunks of FC: all sums of a chunk are read before any is
+    // written.  The chunks of one point block are consecutive slots of one XCD
+    // (workgroups go to the 8 XCDs in turn), so the field words they share,
+    // one cache line per point on a plane normal to x, come from that XCD's L2.
     constexpr int FC = 4;
-    const int f0 = int(blockIdx.z) * FC;
-    if (t >= int(U.npts) || f0 >= U.nf)
+    const int b = int(blockIdx.x);
+    const int slot = (b & 7) * int(gridDim.x >> 3) + (b >> 3);
+    const int pb = slot / A.nch;
+    const int f0 = (slot - pb * A.nch) * FC;
+    const int t = pb * int(blockDim.x) + int(threadIdx.x);  // unit sizes are checked < 2^31
+    if (pb >= A.npb || t >= int(U.npts) || f0 >= U.nf)
         return;
     const int ii = t % U.dims[0];
     const int r = t / U.dims[0];
@@ -1937,8 +1943,13 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
                 int mf = 1;
                 for (int u = 0; u < D.nunits; ++u)
                     mf = std::max(mf, D.units[u].nf);
-                k_dft_sparse<T>
-                    <<<dim3(unsigned((most + 255) / 256), D.nunits, unsigned((mf + 3) / 4)), 256, 0, stream>>>(D);
+                D.npb = int((most + 255) / 256);
+                D.nch = (mf + 3) / 4;
+                // a multiple of 8 workgroups, so that the XCD remap is a bijection
+                const int64_t nb = (int64_t(D.npb) * D.nch + 7) / 8 * 8;
+                if (nb > 0x7fffffff)
+                    throw std::runtime_error("DFT monitor too large: points x frequency chunks exceed one grid");
+                k_dft_sparse<T><<<dim3(unsigned(nb), D.nunits), 256, 0, stream>>>(D);
             }
             HIP_OK(hipGetLastError());
         }
