@@ -45,12 +45,17 @@ def workload(name):
         return W.voxel_mix(n=512, precision="fp32"), None, (
             "voxel mix 512^3 FP32: 16-cell CPML, Ez Gaussian1 sheet, 8^3-cell voxels, 30% 3-pole Lorentz "
             "(palette of 8), 10% Drude (palette of 4), no DFT")
+    if name == "c5":
+        return W.voxel_mix(n=640, precision="fp32", nfreq=64, nplanes=8), None, (
+            "voxel mix 640^3 FP32: 16-cell CPML, Ez Gaussian1 sheet, 8^3-cell voxels, 30% 3-pole Lorentz "
+            "(palette of 8), 10% Drude (palette of 4), 8 Ez DFT planes at half-integer z x 64 freqs")
     raise SystemExit("unknown workload %r" % name)
 
 
 # workloads too large for the CPU oracle's set-up are sampled on a smaller
 # instance of the same recipe (per-cell rate reported)
-CPU_SAMPLE = {"t512": lambda: __import__("ffip_amd.workloads", fromlist=["voxel_mix"]).voxel_mix(n=128)}
+CPU_SAMPLE = {"t512": lambda: __import__("ffip_amd.workloads", fromlist=["voxel_mix"]).voxel_mix(n=128),
+              "c5": lambda: __import__("ffip_amd.workloads", fromlist=["voxel_mix"]).voxel_mix(n=128, nfreq=64)}
 
 
 def cpu_baseline(cfg, ds, steps, cells):
