@@ -202,3 +202,33 @@ def test_full_size_dispersive_fp32_against_fp64():
         print("monitor %d: fp32 vs fp64 rel L2 %.3e, |DFT| max %.3e" % (k, e, np.abs(da["real"]).max()))
         assert np.abs(da["real"]).max() > 0
         assert e <= 1e-4
+
+
+def test_full_size_voxel_mix_512_fp32_against_fp64():
+    """The 512^3 voxel mix at its real shape (three-pole Lorentz and Drude voxels,
+    compact polarization, CPML interior/shell launches), with two Ez DFT planes
+    added, 300 steps: FP32 storage against FP64 storage on the monitors and on
+    point probes (the FP64 engine equals the oracle at 1e-10 on the small cases)."""
+    cfg = W.voxel_mix(n=512, precision="fp64", nfreq=4, nplanes=2)
+    cfg["stop condition"] = {"type": "time", "time": 150.0}
+    pts = [(0.5, 0.0, -140.0), (30.0, -20.0, -100.0), (-60.0, 45.0, -180.0)]
+    out = {}
+    for prec in ("fp64", "fp32"):
+        cfg["precision"] = prec
+        sim = LibSim(cfg)
+        assert sim.run(cfg["stop condition"]) == 300
+        out[prec] = ([sim.dft(k) for k in range(2)], [sim.probe(p, "Ez") for p in pts])
+        sim.close()
+    (d64, p64), (d32, p32) = out["fp64"], out["fp32"]
+    reached = False
+    for k in range(2):
+        scale = np.abs(d64[k]["real"]).max()
+        reached = reached or scale > 0
+        if scale > 0:
+            e = max(rel_l2(d32[k][part], d64[k][part]) for part in ("real", "imag"))
+            print("monitor %d: fp32 vs fp64 rel L2 %.3e, |DFT| max %.3e" % (k, e, scale))
+            assert e <= 1e-4
+    assert reached
+    p64, p32 = np.array(p64), np.array(p32)
+    assert np.abs(p64).max() > 0
+    assert np.abs(p32 - p64).max() <= 1e-4 * np.abs(p64).max()
