@@ -1390,8 +1390,10 @@ void EngineImpl::build()
             const size_t na = size_t(L.alloc);
             std::unique_ptr<uint32_t[]> packed(new uint32_t[na]);
             const uint8_t *i0 = M.ids[0].data(), *i1 = M.ids[1].data(), *i2 = M.ids[2].data();
-            for (size_t q = 0; q < na; ++q)
-                packed[q] = uint32_t(i0[q]) | uint32_t(i1[q]) << 8 | uint32_t(i2[q]) << 16;
+            parallel_planes(0, int(L.nplanes) - 1, [&](int p) {
+                for (size_t q = size_t(p) * L.plane, e = q + size_t(L.plane); q < e; ++q)
+                    packed[q] = uint32_t(i0[q]) | uint32_t(i1[q]) << 8 | uint32_t(i2[q]) << 16;
+            });
             ids[h] = upload(packed.get(), na);
             std::vector<MatEntry> me(M.table.eps.size());
             for (size_t e = 0; e < me.size(); ++e)

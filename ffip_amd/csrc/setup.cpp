@@ -140,37 +140,6 @@ static Medium json2medium(const Json& j)
 }
 
 // Abstract_Susceptibility, src/medium.cpp:24-31
-// body(k) for every k in [k0, k1], spread over up to 16 host threads; the set-up
-// loops that use it write disjoint planes.
-template <class F>
-static void parallel_planes(int k0, int k1, F&& body)
-{
-    const int n = k1 - k0 + 1;
-    const unsigned hw = std::thread::hardware_concurrency();
-    const int nt = std::max(1, std::min({int(hw ? hw : 1u), 16, n / 2}));
-    if (nt == 1) {
-        for (int k = k0; k <= k1; ++k)
-            body(k);
-        return;
-    }
-    std::atomic<int> next{k0};
-    auto work = [&] {
-        for (int k = next.fetch_add(1); k <= k1; k = next.fetch_add(1))
-            body(k);
-    };
-    std::vector<std::thread> pool;
-    for (int t = 1; t < nt; ++t) {
-        try {
-            pool.emplace_back(work);
-        } catch (const std::system_error&) {
-            break;  // no more threads: the calling thread takes the rest
-        }
-    }
-    work();
-    for (auto& th : pool)
-        th.join();
-}
-
 static void abstract_sus(const Sus& s, double dt, double& c1, double& c2, double& c3)
 {
     double c0 = 2 * s.b2 + s.b1 * dt;
@@ -610,7 +579,7 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
         // owner[cc]: first geometry (list order) strictly containing each point of
         // the box ob[cc] that covers every geometry's bounding box; points outside
         // it belong to no geometry, so only the geometries' extent is scanned
-        std::vector<std::vector<int>> owner(3);
+        std::vector<RawVec<int>> owner(3);
         int ob[3][3][2];
         bool partial[3] = {false, false, false};  // some real points lie outside ob
         for (int cc = 0; cc < 3; ++cc) {
@@ -658,8 +627,8 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
             }
             const int n0 = ub[0][1] - ub[0][0] + 1, n1 = ub[1][1] - ub[1][0] + 1, n2 = ub[2][1] - ub[2][0] + 1;
             partial[cc] = int64_t(n0) * n1 * n2 < int64_t(e0) * e1 * e2;
-            std::vector<int>& own = owner[cc];
-            own.assign(size_t(n0) * n1 * n2, -1);
+            RawVec<int>& own = owner[cc];
+            own.resize(size_t(n0) * n1 * n2);  // each plane is set to -1 by its thread below
             // geometries touching each plane, in list order, so that planes can be
             // filled independently with the first-containing-geometry rule
             std::vector<std::vector<int>> on_plane(static_cast<size_t>(n2));
@@ -669,6 +638,7 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
                         on_plane[size_t(k - ub[2][0])].push_back(int(g));
             parallel_planes(ub[2][0], ub[2][1], [&](int k) {
                 const double z = L.coord(2, k, par[2]);
+                std::fill_n(&own[size_t(k - ub[2][0]) * n1 * n0], size_t(n1) * n0, -1);
                 for (int g : on_plane[size_t(k - ub[2][0])]) {
                     const std::array<int, 6>& r = rng[g];
                     for (int j = r[1]; j <= r[4]; ++j) {
@@ -688,12 +658,20 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
             auto oix = [&](int i, int j, int k) {
                 return (size_t(k - ub[2][0]) * n1 + (j - ub[1][0])) * n0 + (i - ub[0][0]);
             };
-            // dense region bounding box: points owned by two-medium boxes
+            // dense region bounding box: points owned by two-medium boxes, searched
+            // within the union of those geometries' ranges
             DenseRegion& D = HM.dense[cc];
             int bl[3] = {1 << 30, 1 << 30, 1 << 30}, bh[3] = {-1, -1, -1};
-            for (int k = ub[2][0]; k <= ub[2][1]; ++k)
-                for (int j = ub[1][0]; j <= ub[1][1]; ++j)
-                    for (int i = ub[0][0]; i <= ub[0][1]; ++i) {
+            int pl[3] = {1 << 30, 1 << 30, 1 << 30}, ph[3] = {-1, -1, -1};
+            for (size_t g = 0; g < geoms.size(); ++g)
+                if (hit[g] && geoms[g]->per_point())
+                    for (int a = 0; a < 3; ++a) {
+                        pl[a] = std::min(pl[a], rng[g][a]);
+                        ph[a] = std::max(ph[a], rng[g][3 + a]);
+                    }
+            for (int k = pl[2]; k <= ph[2]; ++k)
+                for (int j = pl[1]; j <= ph[1]; ++j)
+                    for (int i = pl[0]; i <= ph[0]; ++i) {
                         int g = own[oix(i, j, k)];
                         if (g >= 0 && geoms[g]->per_point()) {
                             int q[3] = {i, j, k};
@@ -816,22 +794,38 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
                 return geom_id[g];
             return dense_const_id[cc] >= 0 ? dense_const_id[cc] : 255;
         };
+        // the id every point would share: the default's when some real point lies
+        // outside the owner box, else that of the first owner entry
+        for (int cc = 0; cc < 3 && seen < 0; ++cc) {
+            if (partial[cc])
+                seen = bg_id;
+            else if (!owner[cc].empty())
+                seen = id_of(cc, owner[cc][0]);
+        }
         for (int cc = 0; cc < 3 && uniform; ++cc) {
-            if (partial[cc]) {  // real points outside the box take the default medium
-                if (seen < 0)
-                    seen = bg_id;
-                if (seen != bg_id)
-                    uniform = false;
+            if (partial[cc] && seen != bg_id) {  // real points outside the box take the default medium
+                uniform = false;
+                break;
             }
-            for (int g : owner[cc]) {
-                if (!uniform)
-                    break;
-                int id = id_of(cc, g);
-                if (seen < 0)
-                    seen = id;
-                if (id != seen)
-                    uniform = false;
-            }
+            if (owner[cc].empty())
+                continue;
+            std::vector<int> gid(geoms.size());
+            for (size_t g = 0; g < geoms.size(); ++g)
+                gid[g] = id_of(cc, int(g));
+            const RawVec<int>& own = owner[cc];
+            const size_t pn = size_t(ob[cc][0][1] - ob[cc][0][0] + 1) * (ob[cc][1][1] - ob[cc][1][0] + 1);
+            std::atomic<bool> differ{false};
+            parallel_planes(0, ob[cc][2][1] - ob[cc][2][0], [&](int k) {
+                if (differ.load(std::memory_order_relaxed))
+                    return;
+                const int* q = own.data() + size_t(k) * pn;
+                for (size_t e = 0; e < pn; ++e)
+                    if ((q[e] < 0 ? bg_id : gid[size_t(q[e])]) != seen) {
+                        differ.store(true, std::memory_order_relaxed);
+                        return;
+                    }
+            });
+            uniform = !differ.load();
         }
         if (uniform && seen >= 0 && HM.table.npoles[seen] != 0)
             uniform = false;
@@ -848,7 +842,7 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
             const int lo0 = L.lo[comp][0], lo1 = L.lo[comp][1], lo2 = L.lo[comp][2];
             const int e0 = L.hi[comp][0] - lo0 + 1, e1 = L.hi[comp][1] - lo1 + 1, e2 = L.hi[comp][2] - lo2 + 1;
             if (S.keep_geom_index) {
-                S.geom_index[comp] = owner[cc];
+                S.geom_index[comp].assign(owner[cc].begin(), owner[cc].end());
                 for (int a = 0; a < 3; ++a) {
                     S.geom_box[comp][a][0] = ob[cc][a][0];
                     S.geom_box[comp][a][1] = ob[cc][a][1];
@@ -857,10 +851,12 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
             if (uniform)
                 continue;
             // padding cells keep id 0; real points outside ob take the default medium
-            HM.ids[cc].assign(L.alloc, 0);
+            HM.ids[cc].resize(size_t(L.alloc));
+            uint8_t* ids = HM.ids[cc].data();
+            parallel_planes(0, int(L.nplanes) - 1,
+                            [&](int p) { std::memset(ids + size_t(p) * L.plane, 0, size_t(L.plane)); });
             if (e0 <= 0 || e1 <= 0 || e2 <= 0)
                 continue;
-            uint8_t* ids = HM.ids[cc].data();
             parallel_planes(lo2, lo2 + e2 - 1, [&](int k) {
                 for (int j = lo1; j < lo1 + e1; ++j)
                     std::memset(ids + L.index(lo0, j, k), bg_id, size_t(e0));

@@ -16,15 +16,77 @@
 // zero ghost shell.
 #pragma once
 
+#include <algorithm>
 #include <array>
+#include <atomic>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "json.hpp"
 
 namespace ffip_amd {
+
+// body(k) for every k in [k0, k1], spread over up to 16 host threads (the calling
+// thread takes the rest if no more can start); callers write disjoint ranges.
+template <class F>
+void parallel_planes(int k0, int k1, F&& body)
+{
+    const int n = k1 - k0 + 1;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nt = std::max(1, std::min({int(hw ? hw : 1u), 16, n / 2}));
+    if (nt == 1) {
+        for (int k = k0; k <= k1; ++k)
+            body(k);
+        return;
+    }
+    std::atomic<int> next{k0};
+    auto work = [&] {
+        for (int k = next.fetch_add(1); k <= k1; k = next.fetch_add(1))
+            body(k);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) {
+        try {
+            pool.emplace_back(work);
+        } catch (const std::system_error&) {
+            break;
+        }
+    }
+    work();
+    for (auto& th : pool)
+        th.join();
+}
+
+// Allocator whose value-less construct leaves elements uninitialised, for
+// per-point arrays that threaded plane loops fill afterwards.
+template <class X>
+struct UninitAlloc : std::allocator<X> {
+    template <class U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() = default;
+    template <class U>
+    UninitAlloc(const UninitAlloc<U>&) noexcept
+    {
+    }
+    template <class U>
+    void construct(U*) noexcept
+    {
+    }
+    template <class U, class... Args>
+    void construct(U* p, Args&&... args)
+    {
+        ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...);
+    }
+};
+template <class X>
+using RawVec = std::vector<X, UninitAlloc<X>>;
 
 // Coord_Type bit patterns, include/utility.hpp:48-64 (reference)
 enum CType : int {
@@ -105,7 +167,7 @@ struct DenseRegion {  // per-point media of two-medium boxes (one per component)
 struct HalfMaterial {
     bool uniform = true;
     double u_inv = 1, u_eps = 1;
-    std::vector<uint8_t> ids[3];  // per component, Layout::alloc entries
+    RawVec<uint8_t> ids[3];  // per component, Layout::alloc entries
     MatTable table;
     DenseRegion dense[3];
     int max_poles = 0;
