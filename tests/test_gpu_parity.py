@@ -153,6 +153,17 @@ def test_magnetic_media_and_flux_density_monitors():
     _compare(cfg, probes=probes, steps=300, tol=1e-4, label="magnetic-fp32")
 
 
+def test_many_frequency_monitors_match_oracle():
+    """Monitors with 9 frequencies (three chunks of the general DFT kernel, the
+    last one partial) on planes normal to x, y and z, each spanning several
+    256-point blocks, against the oracle."""
+    freqs = list(np.linspace(0.05, 0.15, 9))
+    cfg = W.vacuum_dipole(n=32, steps=120, pml_cells=6, freqs=freqs, plane_z=5)
+    cfg["fields output"] += [W.dft((3, 0, 0), (0, 32, 32), freqs, "Ez", "volume fields dft 1"),
+                             W.dft((0, -2.5, 0), (32, 0, 32), freqs, "Ex", "volume fields dft 2")]
+    _compare(cfg, label="9-frequency monitors")
+
+
 def test_continuation_equals_single_run():
     cfg = W.vacuum_dipole(n=20, steps=100, pml_cells=4, plane_z=3)
     a = LibSim(cfg)
