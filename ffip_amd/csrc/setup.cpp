@@ -669,18 +669,38 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
                         pl[a] = std::min(pl[a], rng[g][a]);
                         ph[a] = std::max(ph[a], rng[g][3 + a]);
                     }
-            for (int k = pl[2]; k <= ph[2]; ++k)
-                for (int j = pl[1]; j <= ph[1]; ++j)
-                    for (int i = pl[0]; i <= ph[0]; ++i) {
-                        int g = own[oix(i, j, k)];
-                        if (g >= 0 && geoms[g]->per_point()) {
-                            int q[3] = {i, j, k};
-                            for (int a = 0; a < 3; ++a) {
-                                bl[a] = std::min(bl[a], q[a]);
-                                bh[a] = std::max(bh[a], q[a]);
+            if (ph[2] >= pl[2]) {
+                std::vector<char> pp(geoms.size());
+                for (size_t g = 0; g < geoms.size(); ++g)
+                    pp[g] = geoms[g]->per_point() ? 1 : 0;
+                // per plane: x and y extent of the per-point-medium points (empty: lo > hi)
+                const int np = ph[2] - pl[2] + 1;
+                std::vector<std::array<int, 4>> ext(size_t(np), {1 << 30, -1, 1 << 30, -1});
+                parallel_planes(pl[2], ph[2], [&](int k) {
+                    std::array<int, 4>& e = ext[size_t(k - pl[2])];
+                    for (int j = pl[1]; j <= ph[1]; ++j)
+                        for (int i = pl[0]; i <= ph[0]; ++i) {
+                            const int g = own[oix(i, j, k)];
+                            if (g >= 0 && pp[size_t(g)]) {
+                                e[0] = std::min(e[0], i);
+                                e[1] = std::max(e[1], i);
+                                e[2] = std::min(e[2], j);
+                                e[3] = std::max(e[3], j);
                             }
                         }
-                    }
+                });
+                for (int k = pl[2]; k <= ph[2]; ++k) {
+                    const std::array<int, 4>& e = ext[size_t(k - pl[2])];
+                    if (e[1] < e[0])
+                        continue;
+                    bl[0] = std::min(bl[0], e[0]);
+                    bh[0] = std::max(bh[0], e[1]);
+                    bl[1] = std::min(bl[1], e[2]);
+                    bh[1] = std::max(bh[1], e[3]);
+                    bl[2] = std::min(bl[2], k);
+                    bh[2] = std::max(bh[2], k);
+                }
+            }
             // per-point media that are provably the same everywhere (mu = 1 on the
             // magnetic side, typically) collapse to a table entry without evaluation
             bool fast_const = bh[0] >= 0, have_val = false;
