@@ -58,17 +58,29 @@ CPU_SAMPLE = {"t512": lambda: __import__("ffip_amd.workloads", fromlist=["voxel_
               "c5": lambda: __import__("ffip_amd.workloads", fromlist=["voxel_mix"]).voxel_mix(n=128, nfreq=64)}
 
 
-def cpu_baseline(cfg, ds, steps, cells):
-    """Time the oracle's C hot loop (single thread, FP64) for `steps` steps after
-    its set-up, on the same configuration."""
-    from oracle.oracle import OracleSim
+def cpu_threads():
+    """Host threads for the CPU baseline: OMP_NUM_THREADS, else the cores, at most 16."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline(cfg, ds, steps, cells, threads):
+    """Time the oracle's C hot loop (FP64) for `steps` steps after its set-up, on
+    the same configuration, first on one thread and then on `threads`
+    (Mcell-updates/s for each)."""
+    from oracle.oracle import OracleSim, set_threads
     cfg = dict(cfg)
     cfg["precision"] = "fp64"
     sim = OracleSim(cfg, ds, strict_pool=False)
-    t0 = time.perf_counter()
-    sim.run_steps(steps)
-    dt = time.perf_counter() - t0
-    return cells * steps / dt / 1e6
+    rates = []
+    for n in (1, threads):
+        set_threads(n)
+        t0 = time.perf_counter()
+        sim.run_steps(steps)
+        rates.append(cells * steps / (time.perf_counter() - t0) / 1e6)
+    return rates
 
 
 def design_gradient(steps):
@@ -175,20 +187,23 @@ def main():
 
     cpu = None
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        nth = cpu_threads()
         if args.workload in CPU_SAMPLE:
             scfg = CPU_SAMPLE[args.workload]()
             scells = 1
             for s in scfg["size"]:
                 scells *= int(round(s * scfg["resolution"]))
-            v = cpu_baseline(scfg, None, args.cpu_steps, scells)
+            v1, vn = cpu_baseline(scfg, None, args.cpu_steps, scells, nth)
             what = "the same recipe at %d^3 (the full grid exceeds the oracle's set-up memory)" % round(
                 scells ** (1 / 3))
         else:
-            v = cpu_baseline(cfg, ds, args.cpu_steps, cells)
+            v1, vn = cpu_baseline(cfg, ds, args.cpu_steps, cells, nth)
             what = "same workload and grid"
-        cpu = {"value": round(v, 3), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-               "sample": "%s, %d steps after set-up, FP64 oracle (C restatement of the reference hot loop), "
-                         "1 thread on %s" % (what, args.cpu_steps, platform.processor() or platform.machine())}
+        cpu = {"value": round(vn, 3), "unit": "Mcell-updates/s", "cores": nth, "kind": "port",
+               "value_1_thread": round(v1, 3),
+               "sample": "%s, %d steps after set-up on 1 thread and then %d on %d threads, FP64 oracle "
+                         "(C restatement of the reference hot loop, OpenMP over each phase's points) on %s"
+                         % (what, args.cpu_steps, args.cpu_steps, nth, platform.processor() or platform.machine())}
 
     line = {"metric": "Mcell-updates/s", "value": round(value, 1), "unit": "Mcell-updates/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),

@@ -27,11 +27,18 @@
  * in tests/test_oracle_pins.py: discrete plane-wave dispersion in a periodic box,
  * the discrete susceptibility closed form, the CPML geometric series, the DFT of a
  * known series, the Gaussian1 closed form and the reference's Yee3 known answers.
+ *
+ * Threads (OpenMP, osim_set_threads): the per-point loops of each phase write
+ * distinct points, as one MPI rank's loops do, so any thread count gives the
+ * same bits as one.  The dipole list, which may name a point twice, stays serial.
  */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 static const double PI = 3.141592653589793e+00; /* src/utility.cpp:5 */
 
@@ -123,6 +130,7 @@ static void pml_step(OSim *s, int half)
         OPmlList *L = &s->pml[half][i];
         long long st1 = sgn * s->stride[(i + 1) % 3];
         long long st2 = sgn * s->stride[(i + 2) % 3];
+#pragma omp parallel for schedule(static)
         for (int64_t q = 0; q < L->n; ++q) {
             int64_t p = L->idx[q];
             double curl1 = (eh[p + st1] - eh[p - st1]) * s->inv_dx;
@@ -144,6 +152,7 @@ static void curl_step(OSim *s, int half)
         OCurlList *L = &s->curl[half][i];
         long long st1 = sgn * s->stride[(i + 1) % 3];
         long long st2 = sgn * s->stride[(i + 2) % 3];
+#pragma omp parallel for schedule(static)
         for (int64_t q = 0; q < L->n; ++q) {
             int64_t p = L->idx[q];
             double curl1 = (eh[p + st1] - eh[p - st1]) * s->inv_dx;
@@ -173,6 +182,7 @@ static void medium_step(OSim *s, int half)
     for (int g = 0; g < s->ngroups[half]; ++g) {
         OMedGroup *G = &s->groups[half][g];
         const int np = G->npoles;
+#pragma omp parallel for schedule(static)
         for (int64_t q = 0; q < G->npts; ++q) {
             int64_t index = G->idx[q];
             double *p = G->p + q * np;
@@ -214,6 +224,7 @@ static void dft_update(OSim *s, int half, double time)
         ODftUnit *U = &s->units[half][u];
         if (U->n == 0)
             continue;
+#pragma omp parallel for schedule(static)
         for (int64_t q = 0; q < U->n; ++q)
             U->buf[q] = U->is_db ? s->accdb[U->idx[q]] * s->dt : s->eh[U->idx[q]];
         for (int f = 0; f < U->nf; ++f) {
@@ -221,6 +232,7 @@ static void dft_update(OSim *s, int half, double time)
             double sin_wt = -sin(2 * PI * U->freqs[f] * time);
             double *re = U->re + (int64_t)f * U->n;
             double *im = U->im + (int64_t)f * U->n;
+#pragma omp parallel for schedule(static)
             for (int64_t q = 0; q < U->n; ++q) {
                 im[q] += sin_wt * U->buf[q];
                 re[q] += cos_wt * U->buf[q];
@@ -288,6 +300,25 @@ int64_t osim_run_until_time(OSim *s, double time)
         ++taken;
     }
     return taken;
+}
+
+/* threads of the per-point loops (1 when built without OpenMP) */
+void osim_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int osim_get_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 size_t osim_sizeof(void) { return sizeof(OSim); }

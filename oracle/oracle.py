@@ -403,7 +403,18 @@ def _lib():
         _LIB.osim_run_until_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
         _LIB.osim_run_until_time.restype = ctypes.c_int64
         _LIB.osim_sizeof.restype = ctypes.c_size_t
+        _LIB.osim_set_threads.argtypes = [ctypes.c_int]
+        _LIB.osim_get_threads.restype = ctypes.c_int
     return _LIB
+
+
+def set_threads(n):
+    """Threads of the C hot loop's per-point loops (same bits at any count)."""
+    _lib().osim_set_threads(int(n))
+
+
+def get_threads():
+    return int(_lib().osim_get_threads())
 
 
 P = ctypes.POINTER
