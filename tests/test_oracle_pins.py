@@ -244,3 +244,22 @@ def test_two_medium_box_media():
     # Ex points strictly inside |x|<2, |y|<2, |z|<2 (doubled |.|<4): x odd in {-3..3} (4),
     # y, z even in {-2, 0, 2} (3 each) -> 36 points per E component
     assert int(np.sum(ginf == 6.5)) == 3 * 36
+
+
+def test_threads_give_the_same_bits():
+    """The C loop's OpenMP split writes distinct points per phase: any thread count
+    reproduces the single-thread run bit for bit (CPML, curl, two-pole media,
+    Drude, DFT monitors)."""
+    cfg = W.dispersive_spheres(n=40, nspheres=4, steps=30, nfreq=3, pml_cells=6, precision="fp64")
+    out = []
+    prev = O.get_threads()
+    for n in (1, 4):
+        O.set_threads(n)
+        s = O.OracleSim(cfg, None, strict_pool=False)
+        s.run(cfg["stop condition"])
+        out.append(([s.dft_output(k)["real"].copy() for k in range(2)],
+                    [s.field(c).copy() for c in ("Ex", "Ez", "Hy")]))
+    O.set_threads(prev)
+    for a, b in zip(out[0][0] + out[0][1], out[1][0] + out[1][1]):
+        assert np.array_equal(a, b)
+    assert any(np.abs(f).max() > 0 for f in out[0][1])
