@@ -3,6 +3,10 @@
 
     python bench.py [--gpus 1] [--steps 4000] [--warmup 20] [--workload dispersive256]
 
+Workloads: dispersive256 (default, FP32 storage), dispersive256-fp64, vacuum64,
+t512 (512^3 voxel mix, FP32) and c5 (640^3 voxel mix with 8 Ez DFT planes x 64
+frequencies, FP32; 19 GB of device memory).
+
 A step is one full FDTD time step (magnetic then electric half step, each one
 fused kernel) over the whole grid, with inputs resident in HBM.  The default
 workload is the dispersive-scattering configuration: 256^3 Yee cells, 12-cell
@@ -118,7 +122,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="dispersive256")
+    ap.add_argument("--workload", default="dispersive256",
+                    choices=["dispersive256", "dispersive256-fp64", "vacuum64", "t512", "c5"])
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gradient-steps", type=int, default=3000)
