@@ -72,7 +72,14 @@ struct DftDev {
     int64_t npts;
     double* re;
     double* im;
+    // FP32 storage, units summed by k_dft_sparse*: FP32 partial sums of the last
+    // few steps, added into re/im every DFT_FLUSH steps and before any read
+    float* pre;
+    float* pim;
 };
+
+// steps between flushes of the FP32 partial DFT sums into the FP64 sums
+constexpr int DFT_FLUSH = 16;
 
 struct PmlDev {
     const int* smap;
@@ -652,7 +659,9 @@ struct SparseDftArgs {
     int npb, nch;  // k_dft_sparse: 256-point blocks and frequency chunks per unit
 };
 
-template <typename T>
+// MODE 0: FP64 sums; 1: FP32 partial sums; 2: partial sums added into the FP64
+// sums (and cleared) together with this step's sample
+template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_dft_sparse(const SparseDftArgs<T> A)
 {
     const DftDev& U = A.units[blockIdx.y];
@@ -678,7 +687,7 @@ __global__ void __launch_bounds__(256) k_dft_sparse(const SparseDftArgs<T> A)
     double* __restrict__ re = U.re + t;
     double* __restrict__ im = U.im + t;
     const int64_t stride = U.npts;
-    {
+    if constexpr (MODE == 0) {
         double a[FC], b[FC];
 #pragma unroll
         for (int q = 0; q < FC; ++q)
@@ -692,12 +701,42 @@ __global__ void __launch_bounds__(256) k_dft_sparse(const SparseDftArgs<T> A)
                 re[(f0 + q) * stride] = a[q] + ph[2 * (f0 + q)] * v;
                 im[(f0 + q) * stride] = b[q] + ph[2 * (f0 + q) + 1] * v;
             }
+    } else {
+        float* __restrict__ pr = U.pre + t;
+        float* __restrict__ pi = U.pim + t;
+        float a[FC], b[FC];
+        double ra[FC], rb[FC];
+#pragma unroll
+        for (int q = 0; q < FC; ++q)
+            if (f0 + q < U.nf) {
+                a[q] = pr[(f0 + q) * stride];
+                b[q] = pi[(f0 + q) * stride];
+                if constexpr (MODE == 2) {
+                    ra[q] = re[(f0 + q) * stride];
+                    rb[q] = im[(f0 + q) * stride];
+                }
+            }
+#pragma unroll
+        for (int q = 0; q < FC; ++q)
+            if (f0 + q < U.nf) {
+                const double x = double(a[q]) + ph[2 * (f0 + q)] * v;
+                const double y = double(b[q]) + ph[2 * (f0 + q) + 1] * v;
+                if constexpr (MODE == 1) {
+                    pr[(f0 + q) * stride] = float(x);
+                    pi[(f0 + q) * stride] = float(y);
+                } else {
+                    re[(f0 + q) * stride] = ra[q] + x;
+                    im[(f0 + q) * stride] = rb[q] + y;
+                    pr[(f0 + q) * stride] = 0.f;
+                    pi[(f0 + q) * stride] = 0.f;
+                }
+            }
     }
 }
 
 // Single-frequency units (the adjoint loop's monitors): PT points per thread,
 // strided by the grid, with every load of a thread issued before any update.
-template <typename T, int PT>
+template <typename T, int PT, int MODE>
 __global__ void __launch_bounds__(256) k_dft_sparse1(const SparseDftArgs<T> A)
 {
     const DftDev& U = A.units[blockIdx.y];
@@ -707,6 +746,7 @@ __global__ void __launch_bounds__(256) k_dft_sparse1(const SparseDftArgs<T> A)
     const T* u = A.u[U.comp];
     const double c = A.ph[2 * U.mf0], s = A.ph[2 * U.mf0 + 1];
     double v[PT], a[PT], b[PT];
+    float fa[PT], fb[PT];
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
         const int t = t0 + p * stride;
@@ -717,17 +757,44 @@ __global__ void __launch_bounds__(256) k_dft_sparse1(const SparseDftArgs<T> A)
             const int kk = r / U.dims[1];
             const int o = (U.box[0][0] + ii) + A.pitch * (U.box[1][0] + jj) + A.plane * (U.box[2][0] + kk);
             v[p] = double(u[o]);
-            a[p] = U.re[t];
-            b[p] = U.im[t];
+            if constexpr (MODE != 1) {
+                a[p] = U.re[t];
+                b[p] = U.im[t];
+            }
+            if constexpr (MODE != 0) {
+                fa[p] = U.pre[t];
+                fb[p] = U.pim[t];
+            }
         }
     }
 #pragma unroll
     for (int p = 0; p < PT; ++p) {
         const int t = t0 + p * stride;
         if (t < n) {
-            U.re[t] = a[p] + c * v[p];
-            U.im[t] = b[p] + s * v[p];
+            if constexpr (MODE == 0) {
+                U.re[t] = a[p] + c * v[p];
+                U.im[t] = b[p] + s * v[p];
+            } else if constexpr (MODE == 1) {
+                U.pre[t] = float(double(fa[p]) + c * v[p]);
+                U.pim[t] = float(double(fb[p]) + s * v[p]);
+            } else {
+                U.re[t] = a[p] + (double(fa[p]) + c * v[p]);
+                U.im[t] = b[p] + (double(fb[p]) + s * v[p]);
+                U.pre[t] = 0.f;
+                U.pim[t] = 0.f;
+            }
         }
+    }
+}
+
+// the FP32 partial DFT sums of one unit added into its FP64 sums, then cleared
+__global__ void __launch_bounds__(256) k_dft_flush(double* re, double* im, float* pre, float* pim, int64_t n)
+{
+    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
+        re[t] += double(pre[t]);
+        im[t] += double(pim[t]);
+        pre[t] = 0.f;
+        pim[t] = 0.f;
     }
 }
 
@@ -1071,6 +1138,10 @@ struct EngineImpl {
     double *d_inv[2][3] = {}, *d_eps[2][3] = {}, *d_amp[2][3] = {}, *d_c[2][3] = {};
     PmlDev pml[2][3] = {};
     std::vector<DftDev> units[2];
+    // FP32 partial DFT sums exist (dft_partial) and hold steps not yet added to
+    // the FP64 sums (dft_dirty)
+    bool dft_partial = false, dft_dirty = false;
+    void flush_dft();
     std::vector<std::vector<double>> host_freqs[2];
     double* freqs_dev[2] = {};
     int MF[2] = {0, 0};
@@ -1482,6 +1553,11 @@ void EngineImpl::build()
                 throw std::runtime_error("DFT monitor with 2^31 or more points");
             d.re = dalloc<double>(size_t(d.npts) * d.nf);
             d.im = dalloc<double>(size_t(d.npts) * d.nf);
+            if (fp32 && !d.fused && d.npts > 0) {
+                d.pre = dalloc<float>(size_t(d.npts) * d.nf);
+                d.pim = dalloc<float>(size_t(d.npts) * d.nf);
+                dft_partial = true;
+            }
             units[h].push_back(d);
         }
         if (units[h].size() > size_t(MAXU))
@@ -1938,9 +2014,16 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
         D.ph = A.ph;
         if (D.nunits > 0) {
             constexpr int PT = 4;
+            // FP32 engines keep partial sums, added into the FP64 sums every DFT_FLUSH steps
+            const int mode = !dft_partial ? 0 : (n % DFT_FLUSH == DFT_FLUSH - 1 ? 2 : 1);
             if (single) {
-                k_dft_sparse1<T, PT>
-                    <<<dim3(unsigned((most + 256 * PT - 1) / (256 * PT)), D.nunits), 256, 0, stream>>>(D);
+                const dim3 g(unsigned((most + 256 * PT - 1) / (256 * PT)), D.nunits);
+                if (mode == 0)
+                    k_dft_sparse1<T, PT, 0><<<g, 256, 0, stream>>>(D);
+                else if (mode == 1)
+                    k_dft_sparse1<T, PT, 1><<<g, 256, 0, stream>>>(D);
+                else
+                    k_dft_sparse1<T, PT, 2><<<g, 256, 0, stream>>>(D);
             } else {
                 int mf = 1;
                 for (int u = 0; u < D.nunits; ++u)
@@ -1951,7 +2034,13 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
                 const int64_t nb = (int64_t(D.npb) * D.nch + 7) / 8 * 8;
                 if (nb > 0x7fffffff)
                     throw std::runtime_error("DFT monitor too large: points x frequency chunks exceed one grid");
-                k_dft_sparse<T><<<dim3(unsigned(nb), D.nunits), 256, 0, stream>>>(D);
+                const dim3 g(unsigned(nb), D.nunits);
+                if (mode == 0)
+                    k_dft_sparse<T, 0><<<g, 256, 0, stream>>>(D);
+                else if (mode == 1)
+                    k_dft_sparse<T, 1><<<g, 256, 0, stream>>>(D);
+                else
+                    k_dft_sparse<T, 2><<<g, 256, 0, stream>>>(D);
             }
             HIP_OK(hipGetLastError());
         }
@@ -1995,7 +2084,24 @@ void EngineImpl::step_once(hipEvent_t* ev, bool timed_e, bool timed_h)
         k_probes<T><<<(nprobes + 63) / 64, 64, 0, stream>>>(P, n);
         HIP_OK(hipGetLastError());
     }
+    if (dft_partial)
+        dft_dirty = n % DFT_FLUSH != DFT_FLUSH - 1;
     ++n;
+}
+
+void EngineImpl::flush_dft()
+{
+    if (!dft_dirty)
+        return;
+    for (int h = 0; h < 2; ++h)
+        for (auto& U : units[h])
+            if (U.pre) {
+                const int64_t m = U.npts * U.nf;
+                const int64_t nb = std::min<int64_t>((m + 255) / 256, 4096);
+                k_dft_flush<<<unsigned(nb), 256, 0, stream>>>(U.re, U.im, U.pre, U.pim, m);
+                HIP_OK(hipGetLastError());
+            }
+    dft_dirty = false;
 }
 
 void EngineImpl::step_n(int64_t nsteps)
@@ -2108,6 +2214,7 @@ void Engine::dft_dims(int k, int64_t dims[4]) const
 // output_fields_collective (src/dft.cpp:273-320) for one rank
 void Engine::get_dft(int k, double* re, double* im, double* x, double* y, double* z, double* f)
 {
+    p_->flush_dft();
     sync();
     const Setup& s = *p_->S;
     int64_t dims[4];
@@ -2306,6 +2413,8 @@ void Engine::adjoint_gradient(const Engine& adj, const int fwd_out[3], const int
     A.nr = norm[0];
     A.ni = norm[1];
     const int64_t nn = dim[0] * dim[1] * dim[2];
+    F.flush_dft();
+    adj.p_->flush_dft();
     sync();
     HIP_OK(hipStreamSynchronize(Ad.stream));  // its DFT sums are read on this engine's stream
     double* g = F.sample_buf(size_t(nn));
@@ -2396,6 +2505,7 @@ int64_t Engine::run_until_dft(const double c[3], const double s[3], int ct, doub
         const int64_t k = steps_until(P, turnoff + extension * interval);
         double* buf = P.sample_buf(size_t(k));
         auto vals = window_samples(P, k, buf, [&](int64_t q) {
+            P.flush_dft();
             k_dft_norm<<<1, 1024, 0, P.stream>>>(D.re + fi * D.npts, D.im + fi * D.npts, D.npts, buf, q);
             HIP_OK(hipGetLastError());
         });

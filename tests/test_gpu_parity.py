@@ -164,6 +164,34 @@ def test_many_frequency_monitors_match_oracle():
     _compare(cfg, label="9-frequency monitors")
 
 
+@pytest.mark.parametrize("nf", [9, 1])
+def test_fp32_partial_dft_sums_are_flushed(nf):
+    """FP32 engines sum E/H monitors in FP32 over 16 steps before adding into the
+    FP64 sums.  120 steps leave 8 steps in the partial sums when they are read:
+    FP32 must still match FP64 closely (a lost partial would miss 8 steps of a
+    pulse that is still passing), for the general and single-frequency kernels,
+    read once mid-run and once at the end."""
+    freqs = list(np.linspace(0.05, 0.15, nf)) if nf > 1 else [0.1]
+    cfg = W.vacuum_dipole(n=32, steps=120, pml_cells=6, freqs=freqs, plane_z=5)
+    cfg["fields output"] += [W.dft((3, 0, 0), (0, 32, 32), freqs, "Ez", "volume fields dft 1")]
+    got = {}
+    for prec in ("fp64", "fp32"):
+        cfg["precision"] = prec
+        sim = LibSim(cfg)
+        sim.run({"type": "time", "time": 37.5})  # 75 steps: 11 left in the partial sums
+        mid = [sim.dft(k) for k in range(2)]
+        sim.run(cfg["stop condition"])
+        got[prec] = (mid, [sim.dft(k) for k in range(2)])
+        sim.close()
+    for stage in range(2):
+        for k in range(2):
+            a, b = got["fp64"][stage][k], got["fp32"][stage][k]
+            assert np.abs(a["real"]).max() > 0
+            for part in ("real", "imag"):
+                e = rel_l2(b[part], a[part])
+                assert e <= 1e-5, (nf, stage, k, part, e)
+
+
 def test_continuation_equals_single_run():
     cfg = W.vacuum_dipole(n=20, steps=100, pml_cells=4, plane_z=3)
     a = LibSim(cfg)
