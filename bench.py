@@ -12,7 +12,8 @@ fused kernel) over the whole grid, with inputs resident in HBM.  The default
 workload is the dispersive-scattering configuration: 256^3 Yee cells, 12-cell
 CPML, a planar Gaussian current sheet, 32 seeded spheres (16 two-pole Lorentz,
 16 Drude), two DFT planes x 16 frequencies, fields and polarizations stored in
-FP32 with FP64 arithmetic, coefficients and DFT sums.
+FP32 with FP64 arithmetic, coefficients and DFT sums (monitors gather 16 steps in
+FP32 partial sums before adding into the FP64 sums).
 
 Prints one JSON line.  `roofline` prices the dominant kernel's algorithmic bytes
 (ffip_sim_traffic) against its HIP-event duration over the timed region;

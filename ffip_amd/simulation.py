@@ -234,7 +234,8 @@ class Simulation:
     """A simulation set-up; ``run`` executes it on the GPU (ffip/simulation.py:286-464).
 
     Extra keyword ``precision``: "fp64" (default, the reference's arithmetic) or
-    "fp32" (fields and polarizations stored in FP32, FP64 arithmetic and DFT sums).
+    "fp32" (fields and polarizations stored in FP32, FP64 arithmetic and DFT sums;
+    E/H monitors gather 16 steps in FP32 partial sums before adding into them).
     """
 
     def __init__(self, size, resolution, pmls=[], geometry=[], sources=[], symmetry=[], default_material=Medium(),
