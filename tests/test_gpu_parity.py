@@ -1,7 +1,7 @@
 """HIP engine (through the C ABI) against the oracle and the golden fixture.
 
 Bars: FP64 storage agrees with the FP64 oracle within 1e-10 relative L2 on every
-DFT monitor and probe trace; axes and frequencies are
+DFT monitor and probe trace (the FP64 parity bar); axes and frequencies are
 bit-identical.  FP32 storage (FP64 arithmetic, coefficients and DFT sums) is held
 to the per-case tolerances written below and its error is printed."""
 import math
