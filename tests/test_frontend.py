@@ -111,3 +111,22 @@ def test_stop_conditions_match_oracle(stop):
     assert n_g == n_o
     a, b = g.dft(0), o.dft_output(0)
     assert rel_l2(a["real"], b["real"]) <= 1e-10 and rel_l2(a["imag"], b["imag"]) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_dft_stop_condition_fp32_partial_sums():
+    """FP32 storage: the DFT-decay stop condition reads the monitor after adding
+    the FP32 partial sums into the FP64 sums each step of its windows.  Same stop
+    step as the FP64 oracle (the last window's variation, 9.97e-4 against 1e-3,
+    is far outside FP32 rounding) and the final monitor within 1e-5."""
+    stop = {"type": "dft", "center": [0, 0, 3], "size": [16, 16, 0], "field component": "Ez",
+            "time interval examined": 10.0, "variation": 1e-3, "frequency": 0.1}
+    cfg = W.vacuum_dipole(n=16, pml_cells=4, plane_z=3)
+    o = OracleSim(cfg)
+    n_o = o.run(stop)
+    from ffip_amd._lib import LibSim
+    cfg["precision"] = "fp32"
+    g = LibSim(cfg)
+    assert g.run(stop) == n_o
+    a, b = g.dft(0), o.dft_output(0)
+    assert rel_l2(a["real"], b["real"]) <= 1e-5 and rel_l2(a["imag"], b["imag"]) <= 1e-5
