@@ -356,6 +356,80 @@ static void vec3(const Json& j, double out[3])
         out[a] = j[a].num();
 }
 
+// General_Medium_Box, src/geometry.cpp:79-107, built as src/simulation.cpp:439-514
+// reads it: the density's zero-padded trilinear interpolant (as Two_Medium_Box)
+// is placed on the sampled density function, taken as uniformly spaced from its
+// first to its last sample, and selects a zero-padded linear interpolation
+// between the sampled media (interpn<1> over Abstract_Medium, field by field).
+struct GeneralMediumGeom : BoxGeom {
+    int dim[3];
+    std::vector<double> rho, rho_fun;
+    // per sample: eps, mu and the pool amplitudes of each half, as field tables
+    std::vector<double> eps, mu;
+    std::vector<std::vector<double>> e_amp, m_amp;  // [pool entry][sample]
+    GeneralMediumGeom(const double c[3], const double s[3], const int d[3], std::vector<double> r,
+                      std::vector<double> rf, const std::vector<AbsMedium>& media)
+        : BoxGeom(c, s), rho(std::move(r)), rho_fun(std::move(rf))
+    {
+        for (int k = 0; k < 3; ++k)
+            dim[k] = d[k];
+        if (dim[0] < 1 || dim[1] < 1 || dim[2] < 1)
+            throw std::runtime_error("Invalid Dimension");
+        if (int64_t(rho.size()) != int64_t(dim[0]) * dim[1] * dim[2])
+            throw std::runtime_error("general medium box: density dataset size does not match dimension");
+        if (rho_fun.empty() || media.size() != rho_fun.size())
+            throw std::runtime_error("general medium box: the density function and the media tables differ in size");
+        const size_t n = media.size();
+        eps.resize(n);
+        mu.resize(n);
+        e_amp.assign(media[0].e_amp.size(), std::vector<double>(n));
+        m_amp.assign(media[0].m_amp.size(), std::vector<double>(n));
+        for (size_t i = 0; i < n; ++i) {
+            eps[i] = media[i].eps;
+            mu[i] = media[i].mu;
+            for (size_t p = 0; p < e_amp.size(); ++p)
+                e_amp[p][i] = media[i].e_amp[p];
+            for (size_t p = 0; p < m_amp.size(); ++p)
+                m_amp[p][i] = media[i].m_amp[p];
+        }
+    }
+    bool per_point() const override { return true; }
+    // position in the sampled density function of the density at (x, y, z)
+    double table_pos(double x, double y, double z) const
+    {
+        double q[3] = {(z - p1[2]) / (p2[2] - p1[2]) * (dim[2] - 1), (y - p1[1]) / (p2[1] - p1[1]) * (dim[1] - 1),
+                       (x - p1[0]) / (p2[0] - p1[0]) * (dim[0] - 1)};
+        int dz[3] = {dim[2], dim[1], dim[0]};
+        const double val = interpn(rho.data(), dz, q, 3);
+        return (val - rho_fun.front()) / (rho_fun.back() - rho_fun.front()) * double(rho_fun.size() - 1);
+    }
+    double field(const std::vector<double>& tab, double xq) const
+    {
+        const int n = int(tab.size());
+        return interpn(tab.data(), &n, &xq, 1);
+    }
+    double half_at(double x, double y, double z, int half, double* amp, size_t np) const override
+    {
+        const double xq = table_pos(x, y, z);
+        const std::vector<std::vector<double>>& a = half == 0 ? e_amp : m_amp;
+        for (size_t p = 0; p < np; ++p)
+            amp[p] = p < a.size() ? field(a[p], xq) : 0.0;
+        return field(half == 0 ? eps : mu, xq);
+    }
+    AbsMedium medium_at(double x, double y, double z) const override
+    {
+        const double xq = table_pos(x, y, z);
+        AbsMedium r;
+        r.eps = field(eps, xq);
+        r.mu = field(mu, xq);
+        for (auto& t : e_amp)
+            r.e_amp.push_back(field(t, xq));
+        for (auto& t : m_amp)
+            r.m_amp.push_back(field(t, xq));
+        return r;
+    }
+};
+
 static std::unique_ptr<Geometry> build_geometry(const Json& g, const std::vector<Sus>* pool, const DatasetMap& ds,
                                                 double dx2)
 {
@@ -364,7 +438,7 @@ static std::unique_ptr<Geometry> build_geometry(const Json& g, const std::vector
     vec3(g.at("center"), c);
     for (auto& v : c)
         v /= dx2;
-    if (t == "box" || t == "two medium box") {
+    if (t == "box" || t == "two medium box" || t == "general medium box") {
         vec3(g.at("size"), s);
         for (auto& v : s)
             v /= dx2;
@@ -387,6 +461,39 @@ static std::unique_ptr<Geometry> build_geometry(const Json& g, const std::vector
             throw std::runtime_error("dataset \"" + name + "\" was not provided");
         return std::make_unique<TwoMediumGeom>(c, s, d, abstract_medium(pool, json2medium(g.at("medium1"))),
                                                abstract_medium(pool, json2medium(g.at("medium2"))), it->second);
+    }
+    if (t == "general medium box") {
+        int d[3] = {g.at("dimension")[0].integer(), g.at("dimension")[1].integer(), g.at("dimension")[2].integer()};
+        auto data = [&](const char* key) -> const std::vector<double>& {
+            const std::string& name = g.at(key).str();
+            auto it = ds.find(name);
+            if (it == ds.end())
+                throw std::runtime_error("dataset \"" + name + "\" was not provided");
+            return it->second;
+        };
+        const std::vector<double>& rho = data("density dataset");
+        const std::vector<double>& rho_fun = data("density function dataset");
+        const std::vector<double>& epsilon = data("epsilon dataset");
+        const std::vector<double>& mu = data("mu dataset");
+        const std::vector<double>& e_amp = data("electric susceptibility amplitudes dataset");
+        const std::vector<double>& m_amp = data("magnetic susceptibility amplitudes dataset");
+        // the medium with its permittivity, permeability and amplitudes replaced
+        // by each sample's (src/simulation.cpp:498-511)
+        Medium m = json2medium(g.at("medium"));
+        const size_t ne = m.e_amp.size(), nm = m.m_amp.size();
+        if (mu.size() < epsilon.size() || e_amp.size() < epsilon.size() * ne || m_amp.size() < epsilon.size() * nm)
+            throw std::runtime_error("general medium box: a medium dataset is shorter than the epsilon dataset");
+        std::vector<AbsMedium> media;
+        for (size_t i = 0; i < epsilon.size(); ++i) {
+            m.eps = epsilon[i];
+            m.mu = mu[i];
+            for (size_t j = 0; j < ne; ++j)
+                m.e_amp[j] = e_amp[i * ne + j];
+            for (size_t j = 0; j < nm; ++j)
+                m.m_amp[j] = m_amp[i * nm + j];
+            media.push_back(abstract_medium(pool, m));
+        }
+        return std::make_unique<GeneralMediumGeom>(c, s, d, rho, rho_fun, media);
     }
     throw std::runtime_error("geometry type \"" + t + "\" is not supported");
 }

@@ -351,9 +351,62 @@ class Two_Medium_Box:
                 "medium1": self.medium1.get_json(), "medium2": self.medium2.get_json()}
 
 
-class General_Medium_Box:
-    """Not provided: no configuration of the MI355X path uses it (density mapped
-    through a 1-D table of media, src/geometry.cpp:78-107)."""
+class General_Medium_Box(Two_Medium_Box):
+    """A density box whose medium is read from a table of media sampled along the
+    density (ffip/geom.py:446-558; src/geometry.cpp:79-107).
 
-    def __init__(self, *args, **kwargs):
-        raise NotImplementedError("General_Medium_Box is not supported by ffip_amd")
+    The medium is epsilon_inf = 1 with the two given electric susceptibilities.
+    `epsilon_fun(rho)` gives the complex permittivity wanted at `frequency` for
+    density rho in [0, 1].  The two susceptibility amplitudes that reproduce it
+    solve a 2 x 2 real system: Re and Im of epsilon - 1 against Re and Im of each
+    susceptibility's unit-amplitude response at `frequency`.  The table holds 1000
+    densities linspace(0, 1) and their amplitudes.  The engine interpolates the
+    density at each point, then the table linearly."""
+
+    TABLE = 1000
+
+    def __init__(self, size=Vector3(), center=Vector3(), dim=Vector3(), frequency=1.0, density=None,
+                 epsilon_fun=None, e_sus=(), suffix="0"):
+        if len(e_sus) != 2:
+            raise ValueError("Currently only support two poles")
+        super().__init__(size=size, center=center, dim=dim, density=density, suffix=suffix)
+        self.frequency = float(frequency)
+        self.e_sus = deepcopy(list(e_sus))
+        self.medium = Medium(epsilon=1, E_susceptibilities=self.e_sus)
+        self.epsilon_fun = deepcopy(epsilon_fun)
+        # unit-amplitude responses (get_epsilon carries the amplitude sigma)
+        unit = [s.get_epsilon(self.frequency) / s.sigma for s in self.e_sus]
+        self.mat = np.array([[np.real(unit[0]), np.real(unit[1])], [np.imag(unit[0]), np.imag(unit[1])]])
+        self.invmat = np.linalg.inv(self.mat)
+        prefix = "general medium box %s " % suffix
+        self.density_dataset = prefix + "density"
+        self.density_fun_dataset = prefix + "density function"
+        self.epsilon_dataset = prefix + "epsilon"
+        self.mu_dataset = prefix + "mu"
+        self.e_sus_amp_dataset = prefix + "e sus amp"
+        self.m_sus_amp_dataset = prefix + "m sus amp"
+
+    def amplitude_table(self):
+        """(densities, amplitudes [n, 2]) of the sampled media."""
+        rho = np.linspace(0, 1, self.TABLE)
+        eps = np.asarray(self.epsilon_fun(rho), dtype=complex)
+        amps = self.invmat @ np.stack((np.real(eps) - 1, np.imag(eps)))
+        return rho, amps.T
+
+    def dump(self, store):
+        rho, amps = self.amplitude_table()
+        store.create_dataset(self.density_dataset, data=self.density.ravel())
+        store.create_dataset(self.density_fun_dataset, data=rho)
+        store.create_dataset(self.epsilon_dataset, data=np.ones(rho.size))
+        store.create_dataset(self.mu_dataset, data=np.ones(rho.size))
+        store.create_dataset(self.e_sus_amp_dataset, data=amps.ravel())
+        store.create_dataset(self.m_sus_amp_dataset, data=np.ones(0))
+
+    def get_json(self):
+        return {"type": "general medium box", "center": self.center.get_json(), "size": self.size.get_json(),
+                "dimension": self.dim.get_json(), "density dataset": self.density_dataset,
+                "density function dataset": self.density_fun_dataset, "epsilon dataset": self.epsilon_dataset,
+                "mu dataset": self.mu_dataset,
+                "electric susceptibility amplitudes dataset": self.e_sus_amp_dataset,
+                "magnetic susceptibility amplitudes dataset": self.m_sus_amp_dataset,
+                "medium": self.medium.get_json()}

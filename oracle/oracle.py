@@ -385,6 +385,41 @@ class TwoMediumBox(Box):
         return eps, mu, ea, ma
 
 
+class GeneralMediumBox(TwoMediumBox):
+    """General_Medium_Box, src/geometry.cpp:79-107 (media built as
+    src/simulation.cpp:439-514): the density interpolant at the point (as
+    Two_Medium_Box) is placed on the sampled density function, taken as uniformly
+    spaced between its first and last samples,
+        q = (rho - f[0]) / (f[-1] - f[0]) * (len(f) - 1),
+    and the medium is the zero-padded linear interpolation (interpn<1>,
+    include/utility.hpp:753-801) of the sampled media at q, field by field."""
+
+    def __init__(self, center, size, dim, rho, rho_fun, media):
+        super().__init__(center, size, dim, None, None, rho)
+        self.rho_fun = np.asarray(rho_fun, dtype=float).ravel()
+        if self.rho_fun.size == 0 or len(media) != self.rho_fun.size:
+            raise ValueError("density function and media tables differ in size")
+        self.tab_eps = np.array([m.eps for m in media], dtype=float)
+        self.tab_mu = np.array([m.mu for m in media], dtype=float)
+        self.tab_ea = np.array([m.e_amp for m in media], dtype=float).reshape(len(media), -1)
+        self.tab_ma = np.array([m.m_amp for m in media], dtype=float).reshape(len(media), -1)
+
+    def media(self, X):
+        v = self.rho_at(X)
+        f = self.rho_fun
+        n = f.size
+        q = (v - f[0]) / (f[-1] - f[0]) * (n - 1)
+
+        def lin(tab):
+            return _interp_level(tab, n, 1, q, (), (q,))
+
+        ea = np.stack([lin(self.tab_ea[:, p]) for p in range(self.tab_ea.shape[1])], axis=1) \
+            if self.tab_ea.shape[1] else np.zeros((q.size, 0))
+        ma = np.stack([lin(self.tab_ma[:, p]) for p in range(self.tab_ma.shape[1])], axis=1) \
+            if self.tab_ma.shape[1] else np.zeros((q.size, 0))
+        return lin(self.tab_eps), lin(self.tab_mu), ea, ma
+
+
 # ----------------------------------------------------------------------------
 # The simulation set-up (src/simulation.cpp:127-188) and the C driver
 # ----------------------------------------------------------------------------
@@ -658,6 +693,25 @@ class OracleSim:
                                           [int(v) for v in g["dimension"]],
                                           self._abstract(Medium.from_json(g["medium1"])),
                                           self._abstract(Medium.from_json(g["medium2"])), rho))
+            elif t == "general medium box":
+                # the medium with each sample's permittivity, permeability and
+                # amplitudes (src/simulation.cpp:498-511)
+                base = Medium.from_json(g["medium"])
+                eps = np.asarray(self.datasets[g["epsilon dataset"]], dtype=float).ravel()
+                mu = np.asarray(self.datasets[g["mu dataset"]], dtype=float).ravel()
+                ea = np.asarray(self.datasets[g["electric susceptibility amplitudes dataset"]], dtype=float).ravel()
+                ma = np.asarray(self.datasets[g["magnetic susceptibility amplitudes dataset"]], dtype=float).ravel()
+                ne, nm = len(base.e_amp), len(base.m_amp)
+                media = []
+                for i in range(eps.size):
+                    m = Medium(eps[i], mu[i])
+                    m.e_sus, m.m_sus = list(base.e_sus), list(base.m_sus)
+                    m.e_amp = [float(ea[i * ne + j]) for j in range(ne)]
+                    m.m_amp = [float(ma[i * nm + j]) for j in range(nm)]
+                    media.append(self._abstract(m))
+                glist.append(GeneralMediumBox([v / dx2 for v in g["center"]], [v / dx2 for v in g["size"]],
+                                              [int(v) for v in g["dimension"]], self.datasets[g["density dataset"]],
+                                              self.datasets[g["density function dataset"]], media))
             else:
                 raise ValueError("geometry type %r is not supported" % t)
         self.geoms = glist

@@ -246,6 +246,41 @@ def test_two_medium_box_media():
     assert int(np.sum(ginf == 6.5)) == 3 * 36
 
 
+def test_general_medium_box_reduces_to_two_medium_box():
+    """A General_Medium_Box whose table holds two media at densities 0 and 1 is
+    the Two_Medium_Box of those media: the table position equals the density and
+    the linear interpolation m1*rho + m2*(1 - rho) is the same arithmetic, so the
+    oracle's general box must equal its (pinned) two-medium box bit for bit, for
+    densities in [0, 1] including both ends."""
+    rng = np.random.default_rng(5)
+    dims = [4, 3, 5]
+    rho = rng.uniform(0, 1, size=dims[0] * dims[1] * dims[2])
+    rho[:3] = [0.0, 1.0, 0.5]
+    m1 = O.AbstractMedium(7.0, 1.5, [0.3, -0.2], [0.1])
+    m2 = O.AbstractMedium(1.0, 1.0, [0.0, 0.05], [0.0])
+    tmb = O.TwoMediumBox([0, 0, 0], [6, 4, 8], dims, m1, m2, rho)
+    gmb = O.GeneralMediumBox([0, 0, 0], [6, 4, 8], dims, rho, [0.0, 1.0], [m2, m1])
+    X = np.concatenate([rng.uniform(-1, 1, size=(200, 3)) * np.array([2.9, 1.9, 3.9]),
+                        np.array([[-3.0, -2.0, -4.0], [3.0, 2.0, 4.0], [0.0, 0.0, 0.0]])])
+    for a, b in zip(gmb.media(X), tmb.media(X)):
+        assert np.array_equal(a, b)
+
+
+def test_general_medium_box_table_reproduces_the_permittivity():
+    """The front end's amplitude table: 1 + sum_i a_i(rho) chi_i(f) = epsilon_fun(rho)
+    at the design frequency (chi_i the unit-amplitude responses), to rounding."""
+    from ffip_amd.geom import General_Medium_Box, LorentzianSusceptibility, Vector3
+    sus = [LorentzianSusceptibility(frequency=0.08, gamma=0.01), LorentzianSusceptibility(frequency=0.2, gamma=0.02)]
+    fun = lambda r: 1 + 2.5 * r + 0.2j * r * (1 - r)  # noqa: E731
+    g = General_Medium_Box(size=Vector3(2, 2, 2), dim=Vector3(2, 2, 2), frequency=0.1, epsilon_fun=fun, e_sus=sus)
+    rho, amps = g.amplitude_table()
+    chi = np.array([s.get_epsilon(0.1) / s.sigma for s in sus])
+    assert rho.size == 1000 and amps.shape == (1000, 2)
+    assert np.max(np.abs(1 + amps @ chi - fun(rho))) < 1e-12
+    with pytest.raises(ValueError):
+        General_Medium_Box(e_sus=sus[:1], epsilon_fun=fun)
+
+
 def test_threads_give_the_same_bits():
     """The C loop's OpenMP split writes distinct points per phase: any thread count
     reproduces the single-thread run bit for bit (CPML, curl, two-pole media,

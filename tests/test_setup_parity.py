@@ -74,6 +74,23 @@ def _cases():
     # boxes, 3-pole Lorentz and Drude palettes, z-normal DFT planes
     c["voxels"] = (W.voxel_mix(n=32, voxel=4, pml_cells=5, nfreq=2, nplanes=2, precision="fp64", source_offset=7),
                    None, [((1.5, -2.0, 3.0), "Ez")])
+    # General_Medium_Box through the front end: two Lorentz poles whose amplitudes
+    # follow a 1000-entry table over the density
+    from ffip_amd.geom import General_Medium_Box, LorentzianSusceptibility, Vector3
+    from ffip_amd.simulation import _DatasetStore
+    gmb = General_Medium_Box(size=Vector3(8, 6, 4), center=Vector3(0.5, 0, 0), dim=Vector3(9, 7, 5), frequency=0.1,
+                             density=np.random.default_rng(11).uniform(0, 1, (5, 7, 9)),
+                             epsilon_fun=lambda r: 1 + 2.5 * r + 0.2j * r * (1 - r),
+                             e_sus=[LorentzianSusceptibility(frequency=0.08, gamma=0.01),
+                                    LorentzianSusceptibility(frequency=0.2, gamma=0.02)])
+    cfg = W.base((18, 14, 12), 3)
+    cfg["geometry"] = [gmb.get_json()]
+    cfg["sources"] = [{"type": "volume source", "center": [-6.0, 0.0, 0.0], "size": [0.0, 10.0, 8.0],
+                       "amplitude": 1.0, "field component": "Ez", "function": W.gaussian1(0.1)}]
+    cfg["fields output"] = [W.dft((6, 0, 0), (0, 10, 8), [0.08, 0.1], "Ez", "a")]
+    store = _DatasetStore()
+    gmb.dump(store)
+    c["general_box"] = (cfg, store.data, [((0.5, 0.5, 0.5), "Ez")])
     return c
 
 
