@@ -7,6 +7,7 @@ The hot path runs in libffip_hip.so (hand-written HIP kernels); there is no CPU
 fallback and the library must be built (``python -m ffip_amd.build``).
 """
 from ._lib import FfipError, LibSim, TO_convolve, TO_convolve_transpose, transpose  # noqa: F401
+from .polygon import check_inside  # noqa: F401
 from .geom import (Box, DeybeSusceptibility, DrudeSusceptibility, General_Medium_Box,  # noqa: F401
                    GeometricObject, LorentzianSusceptibility, Medium, Sphere, Susceptibility, Two_Medium_Box,
                    Vector3, cmp_shape, metadata_to_pts)

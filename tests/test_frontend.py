@@ -130,3 +130,49 @@ def test_dft_stop_condition_fp32_partial_sums():
     assert g.run(stop) == n_o
     a, b = g.dft(0), o.dft_output(0)
     assert rel_l2(a["real"], b["real"]) <= 1e-5 and rel_l2(a["imag"], b["imag"]) <= 1e-5
+
+
+def test_check_inside_polygon():
+    """_ffip.check_inside (app/ffip.cpp:354-388): strictly inside a simple polygon;
+    edges and vertices are outside; decided exactly for double inputs."""
+    sq = np.array([[0, 0], [2, 0], [2, 2], [0, 2]], dtype=float)
+    got = ffip.check_inside(np.array([[1, 1], [0, 1], [2, 2], [3, 1], [1, 0], [1.999999999, 1], [1, 2.0000000001]]), sq)
+    assert got.tolist() == [True, False, False, False, False, True, False]
+    # a slanted edge: on it, and 2^-40 to either side (exact orientation)
+    tri = np.array([[0, 0], [3, 1], [0, 3]], dtype=float)
+    t = 2.0 ** -40
+    assert ffip.check_inside(np.array([[1.5, 0.5], [1.5, 0.5 + t], [1.5, 0.5 - t]]), tri).tolist() == [
+        False, True, False]
+    # a concave polygon (either orientation) against the winding number from summed
+    # angles, on points kept away from the boundary
+    star = np.array([[np.cos(a) * r, np.sin(a) * r]
+                     for a, r in zip(np.linspace(0, 2 * np.pi, 10, endpoint=False), [3, 1.2] * 5)])
+    rng = np.random.default_rng(3)
+    q = rng.uniform(-3.5, 3.5, size=(2000, 2))
+
+    def winding(p, poly):
+        d = poly[None, :, :] - p[:, None, :]
+        ang = np.arctan2(d[..., 1], d[..., 0])
+        step = np.diff(np.concatenate([ang, ang[:, :1]], axis=1), axis=1)
+        step = (step + np.pi) % (2 * np.pi) - np.pi
+        return np.rint(step.sum(axis=1) / (2 * np.pi)) != 0
+
+    def dist_to_edges(p, poly):
+        a, b = poly, np.roll(poly, -1, axis=0)
+        ab = b - a
+        t_ = np.clip(((p[:, None, :] - a[None]) * ab[None]).sum(-1) / (ab * ab).sum(-1)[None], 0, 1)
+        c = a[None] + t_[..., None] * ab[None]
+        return np.sqrt(((p[:, None, :] - c) ** 2).sum(-1)).min(axis=1)
+
+    keep = dist_to_edges(q, star) > 1e-6
+    for poly in (star, star[::-1]):
+        assert np.array_equal(ffip.check_inside(q, poly)[keep], winding(q, poly)[keep])
+    assert 0 < ffip.check_inside(q, star).sum() < q.shape[0]
+    # shapes and the reference's errors
+    assert ffip.check_inside(np.ones((4, 5, 2)), sq).shape == (4, 5)
+    with pytest.raises(ValueError):
+        ffip.check_inside(np.ones(2), sq)
+    with pytest.raises(ValueError):
+        ffip.check_inside(np.ones((3, 3)), sq)
+    with pytest.raises(ValueError):
+        ffip.check_inside(np.ones((3, 2)), np.ones((4, 3)))
