@@ -10,7 +10,7 @@ from ._lib import FfipError, LibSim, TO_convolve, TO_convolve_transpose, transpo
 from .polygon import check_inside  # noqa: F401
 from .geom import (Box, DeybeSusceptibility, DrudeSusceptibility, General_Medium_Box,  # noqa: F401
                    GeometricObject, LorentzianSusceptibility, Medium, Sphere, Susceptibility, Two_Medium_Box,
-                   Vector3, cmp_shape, metadata_to_pts)
+                   Vector3, cmp_shape, getgrid, metadata_to_pts, planar_polygon, sph2cart)
 from .simulation import (PML, Adjoint_Flux, Adjoint_Source, Adjoint_Volume, Fields_DFT, Flux_Box,  # noqa: F401
                          Flux_Region, Geom_Info, Simulation, Symmetry, field_match_objective, run_until_dft,
                          run_until_fields_decay, run_until_time)

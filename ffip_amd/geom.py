@@ -23,6 +23,12 @@ def metadata_to_pts(*args):
     return np.stack(np.meshgrid(*args, indexing="ij"), axis=-1)
 
 
+def sph2cart(az, el, r):
+    """Cartesian (x, y, z) of azimuth az, elevation el, radius r (ffip/geom.py:22-27)."""
+    horiz = r * np.cos(el)
+    return horiz * np.cos(az), horiz * np.sin(az), r * np.sin(el)
+
+
 class Vector3:
     """3-vector with the arithmetic of ffip.Vector3 (ffip/geom.py:32-170)."""
 
@@ -410,3 +416,43 @@ class General_Medium_Box(Two_Medium_Box):
                 "electric susceptibility amplitudes dataset": self.e_sus_amp_dataset,
                 "magnetic susceptibility amplitudes dataset": self.m_sus_amp_dataset,
                 "medium": self.medium.get_json()}
+
+
+def getgrid(center=None, size=None, dim=None):
+    """(z, y, x) node axes linspace(center - size/2, center + size/2, dim)
+    (ffip/geom.py:561-567)."""
+    center = Vector3() if center is None else center
+    size = Vector3() if size is None else size
+    dim = Vector3() if dim is None else dim
+    axes = [np.linspace(c - w / 2, c + w / 2, int(n)) for c, w, n in
+            ((center.x, size.x, dim.x), (center.y, size.y, dim.y), (center.z, size.z, dim.z))]
+    return axes[2], axes[1], axes[0]
+
+
+def planar_polygon(polygon, center=None, height=np.inf):
+    """Density function of a prism (ffip/geom.py:569-590): 1 where a point
+    (req_pts[..., :] = (z, y, x)) lies strictly inside the polygon in the x-y plane,
+    taken relative to `center`, and within height/2 of center.z; else 0.
+    `polygon` is a vertex array (n, 2), or a shapely-style object: `.exterior`
+    (its coordinates are used) or `.geoms` (a union of such polygons)."""
+    from .polygon import check_inside
+    center = Vector3() if center is None else center
+
+    def vertices(g):
+        ext = getattr(g, "exterior", None)
+        if ext is not None:
+            return np.asarray(getattr(ext, "coords", ext), dtype=float)
+        return np.asarray(g, dtype=float)
+
+    parts = list(polygon.geoms) if hasattr(polygon, "geoms") else [polygon]
+
+    def density(req_pts):
+        req_pts = np.asarray(req_pts, dtype=float)
+        xy = np.stack((req_pts[..., 2] - center.x, req_pts[..., 1] - center.y), axis=-1)
+        inside = np.zeros(req_pts.shape[:-1], dtype=bool)
+        for g in parts:
+            inside |= check_inside(xy, vertices(g))
+        inside &= np.abs(req_pts[..., 0] - center.z) < height / 2
+        return inside.astype(int)
+
+    return density

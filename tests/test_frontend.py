@@ -176,3 +176,31 @@ def test_check_inside_polygon():
         ffip.check_inside(np.ones((3, 3)), sq)
     with pytest.raises(ValueError):
         ffip.check_inside(np.ones((3, 2)), np.ones((4, 3)))
+
+
+def test_geometry_helpers():
+    """sph2cart, getgrid and planar_polygon (ffip/geom.py:22-27, 561-590)."""
+    x, y, z = ffip.sph2cart(np.pi / 2, 0.0, 2.0)
+    assert abs(x) < 1e-15 and y == pytest.approx(2.0) and z == 0.0
+    x, y, z = ffip.sph2cart(0.0, np.pi / 6, 2.0)
+    assert (x, y, z) == pytest.approx((np.sqrt(3), 0.0, 1.0))
+    zs, ys, xs = ffip.getgrid(ffip.Vector3(1, 2, 3), ffip.Vector3(4, 2, 6), ffip.Vector3(5, 3, 4))
+    assert np.allclose(xs, np.linspace(-1, 3, 5)) and np.allclose(ys, [1, 2, 3]) and np.allclose(zs, [0, 2, 4, 6])
+    # a 2 x 2 square prism of height 2 around (1, 0, 0), as vertices and as a
+    # shapely-style object with a closing vertex
+    sq = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]], dtype=float)
+
+    class Shape:
+        class exterior:
+            coords = np.vstack([sq, sq[:1]])
+
+    class Multi:
+        geoms = [Shape, sq + 5]
+
+    pts = ffip.metadata_to_pts(np.array([-0.5, 0.5, 1.5]), np.array([0.0]), np.array([0.5, 1.5, 2.5, 6.0]))
+    for poly in (sq, Shape):
+        d = ffip.planar_polygon(poly, center=ffip.Vector3(1, 0, 0), height=2)(pts)
+        assert d.shape == (3, 1, 4) and d.dtype.kind == "i"
+        assert d[:, 0, :].tolist() == [[1, 1, 0, 0], [1, 1, 0, 0], [0, 0, 0, 0]]
+    d = ffip.planar_polygon(Multi, center=ffip.Vector3(1, 0, 0))(pts)
+    assert d[2, 0, :].tolist() == [1, 1, 0, 0]
