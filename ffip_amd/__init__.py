@@ -12,6 +12,6 @@ from .geom import (Box, DeybeSusceptibility, DrudeSusceptibility, General_Medium
                    GeometricObject, LorentzianSusceptibility, Medium, Sphere, Susceptibility, Two_Medium_Box,
                    Vector3, cmp_shape, getgrid, metadata_to_pts, planar_polygon, sph2cart)
 from .simulation import (PML, Adjoint_Flux, Adjoint_Source, Adjoint_Volume, Fields_DFT, Flux_Box,  # noqa: F401
-                         Flux_Region, Geom_Info, Simulation, Symmetry, field_match_objective, run_until_dft,
-                         run_until_fields_decay, run_until_time)
+                         Flux_Region, Geom_Info, Simulation, Symmetry, direction_dict, field_match_objective,
+                         run_until_dft, run_until_fields_decay, run_until_time, side_dict)
 from .source import Gaussian1, Inhom_Source, Source, Source_Function  # noqa: F401
