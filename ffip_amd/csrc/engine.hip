@@ -1408,13 +1408,13 @@ void EngineImpl::build()
     fp32 = s.precision == "fp32";
     tsz = fp32 ? 4 : 8;
     // measured on MI355X (256^3 dispersive, 512^3 voxel-mix and 200x200x100
-    // design workloads, FP32 and FP64): 8-plane z segments; kernels with poles
-    // issue two planes per load batch (one when a point carries more than two
-    // poles, and in FP64 on the CPML tiles of a split half step), the others
-    // two in FP32 and one in FP64
+    // design workloads, FP32 and FP64): 8-plane z segments; FP32 kernels issue
+    // two planes per load batch (one when a point carries more than two poles,
+    // except in the CPML-free kernel), FP64 kernels one (512^3 FP64 E half step
+    // 6151 -> 5630 us against two for the pole kernels)
     lz = 8;
     unroll = fp32 ? 2 : 1;
-    unroll_mat = 2;
+    unroll_mat = fp32 ? 2 : 1;
     unroll_shell = fp32 ? 0 : 1;
     // tuning overrides: z segment per workgroup, planes per load batch
     if (const char* e = std::getenv("FFIP_LZ"))
