@@ -1,19 +1,18 @@
 #!/usr/bin/env python3
 """Throughput of the fused FDTD step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps 4000] [--warmup 20] [--workload dispersive256]
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--workload t512]
 
-Workloads: dispersive256 (default, FP32 storage), dispersive256-fp64, vacuum64,
-t512 (512^3 voxel mix, FP32) and c5 (640^3 voxel mix with 8 Ez DFT planes x 64
-frequencies, FP32; 19 GB of device memory).
+Workloads: t512 (default: 512^3 voxel mix, FP32 storage), c5 (640^3 voxel mix
+with 8 Ez DFT planes x 64 frequencies, FP32; about 21 GB of device memory),
+dispersive256 (FP32), dispersive256-fp64 and vacuum64.
 
-A step is one full FDTD time step (magnetic then electric half step, each one
-fused kernel) over the whole grid, with inputs resident in HBM.  The default
-workload is the dispersive-scattering configuration: 256^3 Yee cells, 12-cell
-CPML, a planar Gaussian current sheet, 32 seeded spheres (16 two-pole Lorentz,
-16 Drude), two DFT planes x 16 frequencies, fields and polarizations stored in
-FP32 with FP64 arithmetic, coefficients and DFT sums (monitors gather 16 steps in
-FP32 partial sums before adding into the FP64 sums).
+A step is one full FDTD time step (magnetic then electric half step) over the
+whole grid, with inputs resident in HBM.  The default workload is the 512^3
+voxel mix: 512^3 Yee cells, 16-cell CPML, a planar Gaussian Ez current sheet
+normal to x, 8^3-cell voxels of which 30% hold a three-pole Lorentz dielectric
+and 10% a Drude metal, fields and polarizations stored in FP32 with FP64
+arithmetic and coefficients.
 
 Prints one JSON line.  `roofline` prices the dominant kernel's algorithmic bytes
 (ffip_sim_traffic) against its HIP-event duration over the timed region;
@@ -64,11 +63,11 @@ CPU_SAMPLE = {"t512": lambda: __import__("ffip_amd.workloads", fromlist=["voxel_
 
 
 def cpu_threads():
-    """Host threads for the CPU baseline: OMP_NUM_THREADS, else the cores, at most 16."""
+    """Host threads for the CPU baseline: the cores this process may run on."""
     try:
-        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
-    except ValueError:
-        return max(1, min(16, os.cpu_count() or 1))
+        return max(1, len(os.sched_getaffinity(0)))
+    except (AttributeError, OSError):
+        return max(1, os.cpu_count() or 1)
 
 
 def cpu_baseline(cfg, ds, steps, cells, threads):
@@ -121,14 +120,15 @@ def design_gradient(steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="dispersive256",
+    ap.add_argument("--workload", default="t512",
                     choices=["dispersive256", "dispersive256-fp64", "vacuum64", "t512", "c5"])
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--gradient-steps", type=int, default=3000)
     ap.add_argument("--no-gradient", action="store_true")
+    ap.add_argument("--no-stream", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine holds the whole grid on a single MI355X and has no "
@@ -180,6 +180,14 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": kname, "avg_launch_us": round(avg * 1e6, 2), "bytes_per_launch": tr["total"]}
     step_bytes = te["total"] + th["total"]
+    if not args.no_stream:
+        # measured ceiling: a streaming kernel that reads two arrays for every one it
+        # writes (the half steps' mix) and a copy, 1 GiB per array
+        from ffip_amd._lib import stream_bandwidth
+        s21 = stream_bandwidth(2, 1, 1 << 30, 10)
+        s11 = stream_bandwidth(1, 1, 1 << 30, 10)
+        roof["measured_ceiling"] = {"read2_write1_gbs": round(s21[0], 1), "copy_gbs": round(s11[0], 1),
+                                    "frac_of_read2_write1": round(achieved / s21[0], 4)}
     detail = {"electric": {"avg_us": round(e_avg * 1e6, 2), "traffic": te},
               "magnetic": {"avg_us": round(h_avg * 1e6, 2), "traffic": th},
               "step_hbm_frac": round(step_bytes / (ms_per_step / 1000) / 1e9 / HBM_PEAK_GBS, 4),

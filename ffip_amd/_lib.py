@@ -57,6 +57,8 @@ _SIGS = {
     "ffip_transpose": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                         ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
     "ffip_sim_destroy": ([ctypes.c_void_p], None),
+    "ffip_stream_bandwidth": ([ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p],
+                              ctypes.c_int),
     "ffip_to_convolve": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
                           ctypes.c_int64, ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
     # surrogate kernels (device pointers, stream as void*)
@@ -103,6 +105,14 @@ def _check(rc):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def stream_bandwidth(reads: int, writes: int, bytes_per_array: int = 1 << 30, reps: int = 10):
+    """(mean, best) GB/s of a plain streaming kernel reading `reads` and writing
+    `writes` arrays of `bytes_per_array` bytes on the current device."""
+    out = np.zeros(2)
+    _check(lib().ffip_stream_bandwidth(int(reads), int(writes), int(bytes_per_array), int(reps), _ptr(out)))
+    return float(out[0]), float(out[1])
 
 
 def transpose(gx, gy, gz, pt):

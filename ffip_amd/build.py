@@ -21,7 +21,7 @@ EXE = os.path.join(HERE, "run_sim_json")
 ARCH = os.environ.get("FFIP_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["json.cpp", "setup.cpp", "capi.cpp"]
-HIP_SOURCES = ["engine.hip", "surrogate.hip", "filters.hip"]
+HIP_SOURCES = ["engine.hip", "surrogate.hip", "filters.hip", "stream.hip"]
 # the file drop-in executable (links the library; libhdf5 is loaded at run time)
 EXE_SOURCES = ["run_sim_json.cpp", "hdf5_io.cpp", "json.cpp"]
 

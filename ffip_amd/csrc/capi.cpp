@@ -16,6 +16,7 @@
 #include "surrogate.hpp"
 
 namespace ffip_amd {
+void stream_bandwidth(int R, int W, size_t bytes, int reps, double out[2]);
 void to_convolve(const double* in, int64_t n0, int64_t n1, const double* filt, int64_t f0, int64_t f1, bool transpose,
                  double* out);
 }
@@ -34,6 +35,11 @@ struct ffip_sim {
 
 static thread_local std::string g_err;
 
+// a call made in the wrong state of the handle (FFIP_ESTATE)
+struct StateError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
 static int fail(int code, const std::string& msg)
 {
     g_err = msg;
@@ -46,6 +52,8 @@ static int guard(F&& f)
     try {
         f();
         return FFIP_OK;
+    } catch (const StateError& e) {
+        return fail(FFIP_ESTATE, e.what());
     } catch (const std::runtime_error& e) {
         std::string m = e.what();
         return fail(m.rfind("HIP error", 0) == 0 ? FFIP_EDEVICE : FFIP_EINVAL, m);
@@ -61,7 +69,7 @@ static Engine& need_engine(ffip_sim* s)
     if (!s)
         throw std::runtime_error("null handle");
     if (!s->engine)
-        throw std::logic_error("ffip_sim_finalize_setup has not been called");
+        throw StateError("ffip_sim_finalize_setup has not been called");
     return *s->engine;
 }
 
@@ -668,6 +676,13 @@ int ffip_to_convolve(const double* in, int64_t n0, int64_t n1, const double* fil
     if (!in || !filt || !out)
         return fail(FFIP_EINVAL, "null argument");
     return guard([&] { to_convolve(in, n0, n1, filt, f0, f1, transpose != 0, out); });
+}
+
+int ffip_stream_bandwidth(int reads, int writes, size_t bytes_per_array, int reps, double out[2])
+{
+    if (!out)
+        return fail(FFIP_EINVAL, "null argument");
+    return guard([&] { stream_bandwidth(reads, writes, bytes_per_array, reps, out); });
 }
 
 // ------------------------------------------------------------ surrogate kernels

@@ -182,8 +182,8 @@ def design_box(grid=(200, 200, 100), pml_cells=10, design=(100, 100, 20), patch=
 
 def voxel_mix(n=512, voxel=8, seed=5, pml_cells=16, steps=10000, precision="fp32", nfreq=0, nplanes=8,
               lorentz_frac=0.3, drude_frac=0.1, source_offset=24):
-    """Large single-GPU workload: n^3 cells, CPML, a Gaussian1 Ez current sheet at
-    z = -n/2 + source_offset, and a seeded mix of voxel-sized boxes in vacuum:
+    """Large single-GPU workload: n^3 cells, CPML, a Gaussian1 Ez current sheet in
+    the plane x = -n/2 + source_offset, and a seeded mix of voxel-sized boxes in vacuum:
     `lorentz_frac` of the voxel^3-cell voxels hold a 3-pole Lorentz dielectric
     drawn from a palette of 8 (eps_inf = 2, f0 in U[0.05,0.2], gamma in
     U[0.005,0.02], delta-eps in U[0.5,2]), `drude_frac` a Drude metal from a
@@ -192,7 +192,12 @@ def voxel_mix(n=512, voxel=8, seed=5, pml_cells=16, steps=10000, precision="fp32
     points on shared faces belong to one of them (first in list order) instead of
     to neither.  With nfreq > 0, `nplanes` Ez DFT planes at half-integer z carry
     nfreq frequencies in [0.05, 0.15].  28 distinct poles: within the
-    reference's 31-pole pool."""
+    reference's 31-pole pool.
+
+    The sheet is normal to x and its current points along z, so it launches a
+    plane wave along x that crosses every z-normal monitor plane (a current
+    normal to its own sheet has no curl away from the sheet edges and radiates
+    nothing)."""
     rng = np.random.default_rng(seed)
     f0 = rng.uniform(0.05, 0.2, size=(8, 3))
     g0 = rng.uniform(0.005, 0.02, size=(8, 3))
@@ -219,8 +224,8 @@ def voxel_mix(n=512, voxel=8, seed=5, pml_cells=16, steps=10000, precision="fp32
         geoms.append({"type": "box", "center": c, "size": [2 * half] * 3, "medium": m})
     cfg = base((n, n, n), pml_cells, precision)
     cfg["geometry"] = geoms
-    cfg["sources"] = [{"type": "volume source", "center": [0.0, 0.0, -n / 2 + source_offset],
-                       "size": [float(n), float(n), 0.0], "amplitude": 1.0, "field component": "Ez",
+    cfg["sources"] = [{"type": "volume source", "center": [-n / 2 + source_offset, 0.0, 0.0],
+                       "size": [0.0, float(n), float(n)], "amplitude": 1.0, "field component": "Ez",
                        "function": gaussian1(0.1)}]
     if nfreq > 0:
         freqs = list(np.linspace(0.05, 0.15, nfreq))

@@ -114,6 +114,14 @@ int ffip_sim_info(const ffip_sim *sim, double out[6]);
 
 void ffip_sim_destroy(ffip_sim *sim);
 
+/* Sustained HBM bandwidth of a plain streaming kernel on the current device:
+ * `reads` arrays (0-3) are read and `writes` arrays (0-2) written, each of
+ * bytes_per_array bytes, 16 bytes per lane per access, `reps` timed launches.
+ * out[0] = mean and out[1] = best rate in GB/s of bytes read plus written.
+ * Allocates and frees its arrays.  No reference counterpart: the measured
+ * ceiling beside which the half-step kernels' byte rates are reported. */
+int ffip_stream_bandwidth(int reads, int writes, size_t bytes_per_array, int reps, double out[2]);
+
 /* Adjoint-method gradient of a two-medium design box on the GPU (the computation
  * of Adjoint_Volume.get_sensitivity, ffip/simulation.py:841-869, with the
  * transpose of app/ffip.cpp:308-352 done as a deterministic gather):

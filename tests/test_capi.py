@@ -74,8 +74,8 @@ def test_state_errors():
     assert rc == 0
     try:
         out = ctypes.c_int64()
-        assert lib().ffip_sim_step(h, 1) == -1 and b"finalize" in lib().ffip_last_error()
-        assert lib().ffip_sim_time_step(h, ctypes.byref(out)) == -1
+        assert lib().ffip_sim_step(h, 1) == -2 and b"finalize" in lib().ffip_last_error()
+        assert lib().ffip_sim_time_step(h, ctypes.byref(out)) == -2
         assert lib().ffip_sim_step(h, -3) == -1
     finally:
         lib().ffip_sim_destroy(h)
