@@ -283,6 +283,12 @@ class LibSim:
         _check(lib().ffip_sim_info(self._h, _ptr(out)))
         return dict(cells=(out[0], out[1], out[2]), device_bytes=out[3], dt=out[4], dx=out[5])
 
+    @property
+    def fused(self) -> bool:
+        """Whether each step runs as one fused kernel (traffic(2) is then defined)."""
+        out = np.zeros(7)
+        return lib().ffip_sim_traffic(self._h, 2, _ptr(out)) == 0
+
     def traffic(self, half: int):
         out = np.zeros(7)
         _check(lib().ffip_sim_traffic(self._h, int(half), _ptr(out)))

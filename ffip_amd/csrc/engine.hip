@@ -98,7 +98,8 @@ struct MatEntry {
 
 template <typename T>
 struct HalfArgs {
-    T* u[3];
+    const T* u[3];  // updated components, values before the half step
+    T* w[3];        // where their new values go (= u, or the other buffer of a fused step)
     const T* s[3];
     int pitch, plane;
     int n[3];
@@ -272,68 +273,88 @@ __device__ __forceinline__ void load_fields(const HalfArgs<T>& A, int o, PlaneIn
 
 // Loads that only some points need: CPML auxiliaries, polarizations, dense
 // 1/eps_inf.  A batch issues all of these before its unconditional loads, so
-// that waiting for the (newer) field data also covers them and the wait counts
-// stay exact whichever of these branches ran.  `id` is the packed material
-// word of the cell and (compact layout) sb[c] the first P entry of its row
-// segment, both fetched one batch earlier.  With the compact layout it must be
-// called by every live lane of the wave (offsets come from ballots over the row).
-template <typename T, bool MAT, int NPM, bool CP, bool DENSE>
+// that waiting for the (newer) field data also covers them.  With ALL_LANES
+// every lane issues every polarization load of its kernel's shape (NPM poles
+// per component) and a point without the pole reads element 0 of the array and
+// ignores it: under a per-point branch the compiler converts the loaded value
+// to FP64 inside the branch, right after its load, and so waits for each load
+// as soon as it is issued, which the fused kernel (loads and use in one plane)
+// cannot hide.  The half-step kernels, whose batches wait for everything at
+// once, measured faster with the branch (512^3 voxel mix, electric half step
+// 2956 vs 3066 us).  Kernels with more pole slots than the half has (NPM =
+// MAXS) always keep the branch.
+// `id` is the packed material word of the cell and (compact layout) sb[c] the
+// first P entry of its row segment, both fetched one batch earlier.  With the
+// compact layout it must be called by every live lane of the wave (offsets
+// come from ballots over the row).
+template <typename T, bool MAT, int NPM, bool CP, bool DENSE, bool PML, bool ALL_LANES>
 __device__ __forceinline__ void load_sparse(const HalfArgs<T>& A, const MatEntry* smat, const ColCtx& X,
                                             const ZCtx& Z, uint32_t id, const uint32_t sb[3], const bool act[3],
                                             PlaneIn<T, NPM, CP>& L)
 {
-    // CPML auxiliaries: component 0 differentiates along (y, z), 1 along (z, x), 2 along (x, y)
-    if (act[0]) {
-        if (X.sy >= 0)
-            L.psi[0][0] = ldb(static_cast<const T*>(A.pml[1].psi1), uint32_t(psi_off_y(A, X, Z.k)) * sizeof(T));
-        if (Z.sz >= 0)
-            L.psi[0][1] = ldb(static_cast<const T*>(A.pml[2].psi2), uint32_t(psi_off_z(A, X, Z.sz)) * sizeof(T));
-    }
-    if (act[1]) {
-        if (Z.sz >= 0)
-            L.psi[1][0] = ldb(static_cast<const T*>(A.pml[2].psi1), uint32_t(psi_off_z(A, X, Z.sz)) * sizeof(T));
-        if (X.sx >= 0)
-            L.psi[1][1] = ldb(static_cast<const T*>(A.pml[0].psi2), uint32_t(psi_off_x(A, X, Z.k)) * sizeof(T));
-    }
-    if (act[2]) {
-        if (X.sx >= 0)
-            L.psi[2][0] = ldb(static_cast<const T*>(A.pml[0].psi1), uint32_t(psi_off_x(A, X, Z.k)) * sizeof(T));
-        if (X.sy >= 0)
-            L.psi[2][1] = ldb(static_cast<const T*>(A.pml[1].psi2), uint32_t(psi_off_y(A, X, Z.k)) * sizeof(T));
+    constexpr uint32_t E = sizeof(T);
+    if constexpr (PML) {
+        // component 0 differentiates along (y, z), 1 along (z, x), 2 along (x, y).
+        // These stay per-point branches: on the CPML tiles (few points in a slab,
+        // register-bound kernels) they measured faster than issuing all six.
+        if (act[0]) {
+            if (X.sy >= 0)
+                L.psi[0][0] = ldb(static_cast<const T*>(A.pml[1].psi1), uint32_t(psi_off_y(A, X, Z.k)) * E);
+            if (Z.sz >= 0)
+                L.psi[0][1] = ldb(static_cast<const T*>(A.pml[2].psi2), uint32_t(psi_off_z(A, X, Z.sz)) * E);
+        }
+        if (act[1]) {
+            if (Z.sz >= 0)
+                L.psi[1][0] = ldb(static_cast<const T*>(A.pml[2].psi1), uint32_t(psi_off_z(A, X, Z.sz)) * E);
+            if (X.sx >= 0)
+                L.psi[1][1] = ldb(static_cast<const T*>(A.pml[0].psi2), uint32_t(psi_off_x(A, X, Z.k)) * E);
+        }
+        if (act[2]) {
+            if (X.sx >= 0)
+                L.psi[2][0] = ldb(static_cast<const T*>(A.pml[0].psi1), uint32_t(psi_off_x(A, X, Z.k)) * E);
+            if (X.sy >= 0)
+                L.psi[2][1] = ldb(static_cast<const T*>(A.pml[1].psi2), uint32_t(psi_off_y(A, X, Z.k)) * E);
+        }
     }
     if (MAT) {
 #pragma unroll
         for (int C = 0; C < 3; ++C) {
             const int mid = int((id >> (8 * C)) & 255u);
             int np = 0;
+            bool dense_pt = false;
             if (act[C]) {
-                if (!DENSE || mid != 255) {
+                if (!DENSE || mid != 255)
                     np = smat[mid].np;
-                } else {
+                else {
                     np = A.d_np[C];
-                    L.dinv[C] = A.d_inv[C][dense_off(A, C, X.i, X.j, Z.k)];
+                    dense_pt = true;
                 }
             }
+            if (DENSE && (ALL_LANES || dense_pt))
+                L.dinv[C] = A.d_inv[C][dense_pt ? dense_off(A, C, X.i, X.j, Z.k) : 0];
             uint32_t e = CP ? sb[C] : 0u;
 #pragma unroll
             for (int s = 0; s < NPM; ++s) {
+                const bool has = s < np;
                 if constexpr (CP) {
                     // pole s of the row's points in x order: this lane's rank among
                     // those having it, after the entries of the lower poles
                     uint32_t rank, count;
-                    row_rank(s < np, rank, count);
+                    row_rank(has, rank, count);
                     L.poff[C][s] = e + rank;
                     e += count;
-                }
-                if (s < np) {
-                    if constexpr (CP) {
-                        const uint32_t b = L.poff[C][s] * uint32_t(sizeof(T));
-                        L.pc[C][s] = ldb<T>(A.pcur + A.pcomp[C], b);
-                        L.pp[C][s] = ldb<T>(A.pprev + A.pcomp[C], b);
-                    } else {
-                        L.pc[C][s] = ldb<T>(A.pcur + (s * 3 + C) * A.pslice, uint32_t(Z.o) * sizeof(T));
-                        L.pp[C][s] = ldb<T>(A.pprev + (s * 3 + C) * A.pslice, uint32_t(Z.o) * sizeof(T));
+                    if (ALL_LANES || has) {
+                        const uint32_t bo = has ? L.poff[C][s] * E : 0u;
+                        L.pc[C][s] = ldb<T>(A.pcur + A.pcomp[C], bo);
+                        L.pp[C][s] = ldb<T>(A.pprev + A.pcomp[C], bo);
                     }
+                } else if constexpr (ALL_LANES && NPM < MAXS) {
+                    const uint32_t bo = has ? uint32_t(Z.o) * E : 0u;
+                    L.pc[C][s] = ldb<T>(A.pcur + (s * 3 + C) * A.pslice, bo);
+                    L.pp[C][s] = ldb<T>(A.pprev + (s * 3 + C) * A.pslice, bo);
+                } else if (has) {
+                    L.pc[C][s] = ldb<T>(A.pcur + (s * 3 + C) * A.pslice, uint32_t(Z.o) * E);
+                    L.pp[C][s] = ldb<T>(A.pprev + (s * 3 + C) * A.pslice, uint32_t(Z.o) * E);
                 }
             }
         }
@@ -363,7 +384,7 @@ __device__ __forceinline__ double cpml_apply(void* arr, int q, T psi, double b, 
 
 // Update component C of one point from its preloaded plane data: CPML, ADE
 // poles (src/structure.cpp:56-85, in E form), DFT accumulation, store.
-template <typename T, bool MAT, int NPM, bool CP, bool DENSE, int C>
+template <typename T, bool MAT, int NPM, bool CP, bool DENSE, int C, bool DFT = true>
 __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry* smat, const double* spc,
                                             const ColCtx& X, const ZCtx& Z, double d1, double d2, uint32_t id,
                                             const PlaneIn<T, NPM, CP>& L)
@@ -434,7 +455,7 @@ __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry
     }
     const T stored = T(e_new);
     // DFT monitors (k is uniform across the workgroup)
-    const bool in_dft = A.nfu[C] > 0 && Z.k >= A.fbox[C][2][0] && Z.k <= A.fbox[C][2][1] &&
+    const bool in_dft = DFT && A.nfu[C] > 0 && Z.k >= A.fbox[C][2][0] && Z.k <= A.fbox[C][2][1] &&
                         X.i >= A.fbox[C][0][0] && X.i <= A.fbox[C][0][1] && X.j >= A.fbox[C][1][0] &&
                         X.j <= A.fbox[C][1][1];
     for (int m = 0; in_dft && m < A.nfu[C]; ++m) {
@@ -459,7 +480,7 @@ __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry
             U.re[o] += ph[2 * f] * v;
         }
     }
-    stb<T>(A.u[C], uint32_t(Z.o) * sizeof(T), stored);
+    stb<T>(A.w[C], uint32_t(Z.o) * sizeof(T), stored);
 }
 
 // One half step over a (BX x BY) column tile and a z segment of A.lz planes.
@@ -468,7 +489,7 @@ __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry
 // components differentiated along z are carried between planes in registers.
 // PML = false: the launch's tiles lie clear of every CPML slab (no auxiliary
 // code or registers); DENSE = false: no point of this half has a per-point medium.
-template <typename T, bool IS_E, bool MAT, int NPM, bool CP, int U, bool PML, bool DENSE>
+template <typename T, bool IS_E, bool MAT, int NPM, bool CP, int U, bool PML, bool DENSE, bool DFT>
 __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
 {
     // material table and pole coefficients, staged in LDS (sized at launch:
@@ -602,7 +623,7 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
             for (int C = 0; C < 3; ++C)
                 act[q][C] = k <= ke && X.in[C] && k >= A.lo[C][2] && k <= A.hi[C][2];
             if (k <= ke)
-                load_sparse<T, MAT, NPM, CP, DENSE>(A, smat, X, Z[q], idc[q], sbc[q], act[q], L[q]);
+                load_sparse<T, MAT, NPM, CP, DENSE, PML, false>(A, smat, X, Z[q], idc[q], sbc[q], act[q], L[q]);
         }
         // then the loads every thread makes: next batch's material words and
         // segment bases, fields
@@ -628,21 +649,434 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
             if (act[q][0]) {
                 const double d1 = (double(P.v2b) - w2) * inv_dx;
                 const double d2 = (IS_E ? (g1 - a1) : (a1 - g1)) * inv_dx;
-                update_comp<T, MAT, NPM, CP, DENSE, 0>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
+                update_comp<T, MAT, NPM, CP, DENSE, 0, DFT>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
             if (act[q][1]) {
                 const double d1 = (IS_E ? (g0 - a0) : (a0 - g0)) * inv_dx;
                 const double d2 = (double(P.v2a) - w2) * inv_dx;
-                update_comp<T, MAT, NPM, CP, DENSE, 1>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
+                update_comp<T, MAT, NPM, CP, DENSE, 1, DFT>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
             if (act[q][2]) {
                 const double d1 = (double(P.v1a) - (IS_E ? a1 : g1)) * inv_dx;
                 const double d2 = (double(P.v0b) - (IS_E ? a0 : g0)) * inv_dx;
-                update_comp<T, MAT, NPM, CP, DENSE, 2>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
+                update_comp<T, MAT, NPM, CP, DENSE, 2, DFT>(A, smat, spc, X, Z[q], d1, d2, idc[q], P);
             }
             c0 = P.n0;
             c1 = P.n1;
         }
+    }
+}
+
+// ---------------------------------------------------------------- fused full step
+// One launch advances the interior of the grid by a whole time step (step_m then
+// step_e, src/simulation.cpp:748-766) when the magnetic half is a uniform medium
+// with no magnetic sources, B monitors or ghost faces.  A workgroup owns a BX x BY
+// column tile and a z segment [kb, ke] clear of every CPML slab.  Marching up in
+// z, it forms H^{n+1/2} on plane k+1 (from E^n on planes k and k+1) just before
+// E^{n+1} on plane k (from H^{n+1/2} on planes k and k+1), so each field element
+// is read once and written once per step: 12 accesses per cell instead of the 18
+// of two half-step kernels.  E^{n+1} on the tile's last column and row needs
+// H^{n+1/2} one point beyond the tile; the workgroup forms those values itself
+// from E^n, as the neighbouring tile (or the magnetic half-step kernel, on the
+// CPML shell) does when it forms and stores them, and the segment's first H plane
+// is formed the same way by the segment below.  Fields are therefore
+// double-buffered: E^n and H^{n-1/2} are read from one set, E^{n+1} and
+// H^{n+1/2} written to the other, so no workgroup reads what another writes in
+// the same launch.  The fused kernel covers a box of whole tiles clear of every
+// CPML slab; the half-step kernels cover the rest (the CPML shell), the magnetic
+// one before and the electric one after it, so rim values that fall in the shell
+// are read, already new, from the H^{n+1/2} buffer.  Planes of E^n with a
+// one-point rim (three), of H^{n-1/2} and of H^{n+1/2} with the far rim (two
+// each) are staged in LDS, so a single barrier per plane orders them.  The
+// arithmetic and its order are k_half's.
+template <typename T>
+struct StepArgs {
+    HalfArgs<T> e;     // electric half: u = E^n, w = E^{n+1}
+    const T* hin[3];   // H^{n-1/2}
+    T* hout[3];        // H^{n+1/2}
+    int hlo[3][3], hhi[3][3];  // real points of Hx, Hy, Hz
+    double h_inv;      // 1/mu of the uniform magnetic medium
+    // the box of lattice indices the fused kernel covers (whole tiles, clear of
+    // every CPML slab): first and last index per axis, tiles per axis (z tiles of
+    // e.lz planes); the half-step kernels cover the rest of the grid
+    int fo[3], fe[3], fn[3];
+};
+
+// H^{n+1/2} from its curl differences d1, d2 (k_half's magnetic update in a
+// uniform medium outside the CPML)
+template <typename T>
+__device__ __forceinline__ T h_value(const StepArgs<T>& S, double d1, double d2, T hold)
+{
+    const double acc = d1 - d2;
+    return T(double(hold) + S.e.dt * acc * S.h_inv);
+}
+
+// Everything a thread loads for one plane of the fused step: its own E^n and
+// H^{n-1/2}, and the rim points its role asks for (top row j0+BY, right column
+// i0+BX, bottom row j0-1, left column i0-1, two corners).
+template <typename T>
+struct FPlane {
+    T e[3], h[3];
+    T te[3], th[2];  // top (i, j0+BY): Ex, Ey, Ez; Hz, Hx
+    T re[3], rh[2];  // right (i0+BX, j): Ex, Ey, Ez; Hz, Hy
+    T be[2];         // bottom (i, j0-1): Ex, Ez
+    T le[2];         // left (i0-1, j): Ey, Ez
+    T cl, cr;        // Ey (i0-1, j0+BY), Ex (i0+BX, j0-1)
+    uint32_t id;
+    uint32_t sb[3];
+};
+
+template <typename T, bool MAT, int NPM, bool CP, bool DENSE>
+__global__ void __launch_bounds__(BX * BY) k_step(const StepArgs<T> S)
+{
+    const HalfArgs<T>& A = S.e;
+    extern __shared__ double lds_tab[];
+    MatEntry* smat = reinterpret_cast<MatEntry*>(lds_tab);
+    double* spc = lds_tab + 3 * A.nmats;
+    // staged planes: E^n of three planes (x in [i0-1, i0+BX], y in [j0-1, j0+BY]),
+    // H^{n-1/2} and H^{n+1/2} of two (x in [i0, i0+BX], y in [j0, j0+BY])
+    __shared__ T es[3][3][BY + 2][BX + 2];
+    __shared__ T hos[2][3][BY + 1][BX + 1];
+    __shared__ T hs[2][3][BY + 1][BX + 1];
+    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
+    if (MAT) {
+        const int tid = tx + BX * ty;
+        for (int q = tid; q < A.nmats; q += BX * BY)
+            smat[q] = A.mats[q];
+        if (NPM > 0)
+            for (int q = tid; q < 3 * A.npoles; q += BX * BY)
+                spc[q] = A.coef[q];
+        // visible after the first plane's barrier
+    }
+    const int b = int(blockIdx.x);
+    const int slot = (b & 7) * (A.nblocks >> 3) + (b >> 3);
+    if (slot >= A.ntiles)
+        return;  // the whole workgroup
+    const int t = int(A.tiles[slot]);
+    const int ftx = t % S.fn[0];
+    const int tr = t / S.fn[0];
+    const int fty = tr % S.fn[1];
+    const int ftz = tr / S.fn[1];
+    const int i0 = S.fo[0] + ftx * BX, j0 = S.fo[1] + fty * BY;
+    const int gtx = (i0 - A.dlo[0]) / BX;  // tile column in the half step's grid
+    const int i = i0 + tx, j = j0 + ty;
+    const int kb = S.fo[2] + ftz * A.lz;
+    const int ke = min(kb + A.lz - 1, S.fe[2]);
+    const int jt = j0 + BY, ir = i0 + BX;
+    const bool top = ty == BY - 1, right = tx == BX - 1, bot = ty == 0, left = tx == 0;
+    // rims outside the box: their H^{n+1/2} is in the new buffer already
+    const bool top_sh = jt > S.fe[1], right_sh = ir > S.fe[0], above_sh = ke + 1 > S.fe[2];
+    // load coordinates stay in [dlo - 1, dhi + 1], inside every array's padding
+    auto clx = [&](int x) { return min(max(x, A.dlo[0] - 1), A.dhi[0] + 1); };
+    auto cly = [&](int y) { return min(max(y, A.dlo[1] - 1), A.dhi[1] + 1); };
+    auto clz = [&](int z) { return min(max(z, A.dlo[2] - 1), A.dhi[2] + 1); };
+    const int il = clx(i), jl = cly(j), itl = clx(ir), ill = clx(i0 - 1), jtl = cly(jt), jbl = cly(j0 - 1);
+    const int sz = A.plane;
+    auto off = [&](int x, int y, int z) { return uint32_t(A.origin + x + A.pitch * y + sz * z) * uint32_t(sizeof(T)); };
+    const double inv_dx = A.inv_dx;
+    // electric column context (k_half's, no CPML on these tiles)
+    ColCtx X;
+    X.i = i;
+    X.j = j;
+    X.base = A.origin + i + A.pitch * j;
+#pragma unroll
+    for (int C = 0; C < 3; ++C)
+        X.in[C] = i >= A.lo[C][0] && i <= A.hi[C][0] && j >= A.lo[C][1] && j <= A.hi[C][1];
+    X.sx = X.sy = -1;
+    X.bx = X.cx = X.by = X.cy = 0;
+    X.ikx = X.iky = 1;
+    // magnetic points: own (i, j), top rim (i, jt), right rim (ir, j)
+    auto hreal_xy = [&](int C, int x, int y) {
+        return x >= S.hlo[C][0] && x <= S.hhi[C][0] && y >= S.hlo[C][1] && y <= S.hhi[C][1];
+    };
+    auto hreal_z = [&](int C, int z) { return z >= S.hlo[C][2] && z <= S.hhi[C][2]; };
+    bool hown[3];
+#pragma unroll
+    for (int C = 0; C < 3; ++C)
+        hown[C] = hreal_xy(C, i, j);
+    const bool t_hz = top && hreal_xy(2, i, jt), t_hx = top && hreal_xy(0, i, jt);
+    const bool r_hz = right && hreal_xy(2, ir, j), r_hy = right && hreal_xy(1, ir, j);
+
+    // material words: k_half's rule (outside the material box, the default word)
+    const bool mcol = i >= A.mbox[0][0] && i <= A.mbox[0][1] && j >= A.mbox[1][0] && j <= A.mbox[1][1];
+    const uint32_t bg_word = __builtin_amdgcn_readfirstlane(A.bg_word);
+    constexpr bool POL = MAT && NPM > 0 && CP;
+    // rows past the grid (the last tile row's spare threads) read the last row's entry
+    const int seg0 = (min(j, A.dhi[1]) - A.dlo[1]) * A.gx + gtx;
+    const int segplane = A.seg_ny * A.gx;
+
+    auto load_plane = [&](int kk, FPlane<T>& P) {
+        const int kc = clz(kk);
+        const uint32_t o = off(il, jl, kc);
+#pragma unroll
+        for (int C = 0; C < 3; ++C) {
+            P.e[C] = ldb(A.u[C], o);
+            P.h[C] = ldb(S.hin[C], o);
+        }
+        if (MAT) {
+            const bool in = mcol && kc >= A.mbox[2][0] && kc <= A.mbox[2][1];
+            const uint32_t w = ldb(A.ids, in ? uint32_t(A.origin + il + A.pitch * jl + sz * kc) * 4u : 0u);
+            P.id = in ? w : bg_word;
+        }
+        if (POL) {
+            const int kq = min(max(kc, A.dlo[2]), A.dhi[2]);
+#pragma unroll
+            for (int C = 0; C < 3; ++C)
+                P.sb[C] = A.segbase[C][(kq - A.dlo[2]) * segplane + seg0];
+        }
+        if (top) {
+            const uint32_t q = off(il, jtl, kc);
+            if (!top_sh) {
+#pragma unroll
+                for (int C = 0; C < 3; ++C)
+                    P.te[C] = ldb(A.u[C], q);
+            }
+            const T* const* H = top_sh ? S.hout : S.hin;
+            P.th[0] = ldb(H[2], q);
+            P.th[1] = ldb(H[0], q);
+        }
+        if (right) {
+            const uint32_t q = off(itl, jl, kc);
+            if (!right_sh) {
+#pragma unroll
+                for (int C = 0; C < 3; ++C)
+                    P.re[C] = ldb(A.u[C], q);
+            }
+            const T* const* H = right_sh ? S.hout : S.hin;
+            P.rh[0] = ldb(H[2], q);
+            P.rh[1] = ldb(H[1], q);
+        }
+        if (bot) {
+            const uint32_t q = off(il, jbl, kc);
+            P.be[0] = ldb(A.u[0], q);
+            P.be[1] = ldb(A.u[2], q);
+        }
+        if (left) {
+            const uint32_t q = off(ill, jl, kc);
+            P.le[0] = ldb(A.u[1], q);
+            P.le[1] = ldb(A.u[2], q);
+        }
+        if (left && top)
+            P.cl = ldb(A.u[1], off(ill, jtl, kc));
+        if (right && bot)
+            P.cr = ldb(A.u[0], off(itl, jbl, kc));
+    };
+    // stage plane kk: E^n with its rim in es[kk mod 3], H^{n-1/2} with the far rim in hos[kk mod 2]
+    auto write_planes = [&](int kk, const FPlane<T>& P) {
+        T(*E)[BY + 2][BX + 2] = es[(kk + 3) % 3];
+        T(*H)[BY + 1][BX + 1] = hos[kk & 1];
+#pragma unroll
+        for (int C = 0; C < 3; ++C) {
+            E[C][ty + 1][tx + 1] = P.e[C];
+            H[C][ty][tx] = P.h[C];
+        }
+        if (top) {
+            if (!top_sh) {
+#pragma unroll
+                for (int C = 0; C < 3; ++C)
+                    E[C][BY + 1][tx + 1] = P.te[C];
+            }
+            H[2][BY][tx] = P.th[0];
+            H[0][BY][tx] = P.th[1];
+        }
+        if (right) {
+            if (!right_sh) {
+#pragma unroll
+                for (int C = 0; C < 3; ++C)
+                    E[C][ty + 1][BX + 1] = P.re[C];
+            }
+            H[2][ty][BX] = P.rh[0];
+            H[1][ty][BX] = P.rh[1];
+        }
+        if (bot) {
+            E[0][0][tx + 1] = P.be[0];
+            E[2][0][tx + 1] = P.be[1];
+        }
+        if (left) {
+            E[1][ty + 1][0] = P.le[0];
+            E[2][ty + 1][0] = P.le[1];
+        }
+        if (left && top)
+            E[1][BY + 1][0] = P.cl;
+        if (right && bot)
+            E[0][0][BX + 1] = P.cr;
+    };
+    // polarization / dense-medium loads of E^{n+1} plane kk (load_sparse)
+    auto load_sparse_k = [&](int kk, uint32_t id, const uint32_t sb[3], PlaneIn<T, NPM, CP>& L) {
+        ZCtx Z;
+        Z.k = kk;
+        Z.o = X.base + sz * kk;
+        Z.sz = -1;
+        Z.bz = Z.cz = 0;
+        Z.ikz = 1;
+        bool act[3];
+#pragma unroll
+        for (int C = 0; C < 3; ++C)
+            act[C] = X.in[C] && kk >= A.lo[C][2] && kk <= A.hi[C][2];
+        load_sparse<T, MAT, NPM, CP, DENSE, false, true>(A, smat, X, Z, id, sb, act, L);
+    };
+    // H^{n+1/2} of plane kk (own point and rim points) from the staged E^n planes
+    // kk and kk-1 and H^{n-1/2} of plane kk; fills hs, stores the own point when
+    // kk <= ke and returns its values in hn
+    auto h_plane = [&](int kk, T hn[3]) {
+        const T(*E)[BY + 2][BX + 2] = es[(kk + 3) % 3];
+        const T(*Ep)[BY + 2][BX + 2] = es[(kk + 2) % 3];
+        const T(*G)[BY + 1][BX + 1] = hos[kk & 1];
+        T(*H)[BY + 1][BX + 1] = hs[kk & 1];
+        const bool st = kk <= ke;
+        const double ex = double(E[0][ty + 1][tx + 1]), ey = double(E[1][ty + 1][tx + 1]),
+                     ez = double(E[2][ty + 1][tx + 1]);
+        const uint32_t o = uint32_t(X.base + sz * kk) * uint32_t(sizeof(T));
+        {
+            const bool r = hown[0] && hreal_z(0, kk);
+            const double d1 = (double(E[2][ty][tx + 1]) - ez) * inv_dx;
+            const double d2 = (double(Ep[1][ty + 1][tx + 1]) - ey) * inv_dx;
+            const T h0 = G[0][ty][tx];
+            const T v = r ? h_value(S, d1, d2, h0) : h0;
+            hn[0] = v;
+            H[0][ty][tx] = v;
+            if (r && st)
+                stb(S.hout[0], o, v);
+        }
+        {
+            const bool r = hown[1] && hreal_z(1, kk);
+            const double d1 = (double(Ep[0][ty + 1][tx + 1]) - ex) * inv_dx;
+            const double d2 = (double(E[2][ty + 1][tx]) - ez) * inv_dx;
+            const T h0 = G[1][ty][tx];
+            const T v = r ? h_value(S, d1, d2, h0) : h0;
+            hn[1] = v;
+            H[1][ty][tx] = v;
+            if (r && st)
+                stb(S.hout[1], o, v);
+        }
+        {
+            const bool r = hown[2] && hreal_z(2, kk);
+            const double d1 = (double(E[1][ty + 1][tx]) - ey) * inv_dx;
+            const double d2 = (double(E[0][ty][tx + 1]) - ex) * inv_dx;
+            const T h0 = G[2][ty][tx];
+            const T v = r ? h_value(S, d1, d2, h0) : h0;
+            hn[2] = v;
+            H[2][ty][tx] = v;
+            if (r && st)
+                stb(S.hout[2], o, v);
+        }
+        if (top && top_sh) {
+            H[2][BY][tx] = G[2][BY][tx];
+            H[0][BY][tx] = G[0][BY][tx];
+        } else if (top) {
+            const double txv = double(E[0][BY + 1][tx + 1]), tyv = double(E[1][BY + 1][tx + 1]),
+                         tzv = double(E[2][BY + 1][tx + 1]);
+            {  // Hz at (i, jt)
+                const double d1 = (double(E[1][BY + 1][tx]) - tyv) * inv_dx;
+                const double d2 = (double(E[0][BY][tx + 1]) - txv) * inv_dx;
+                const T h0 = G[2][BY][tx];
+                H[2][BY][tx] = t_hz && hreal_z(2, kk) ? h_value(S, d1, d2, h0) : h0;
+            }
+            {  // Hx at (i, jt)
+                const double d1 = (double(E[2][BY][tx + 1]) - tzv) * inv_dx;
+                const double d2 = (double(Ep[1][BY + 1][tx + 1]) - tyv) * inv_dx;
+                const T h0 = G[0][BY][tx];
+                H[0][BY][tx] = t_hx && hreal_z(0, kk) ? h_value(S, d1, d2, h0) : h0;
+            }
+        }
+        if (right && right_sh) {
+            H[2][ty][BX] = G[2][ty][BX];
+            H[1][ty][BX] = G[1][ty][BX];
+        } else if (right) {
+            const double rxv = double(E[0][ty + 1][BX + 1]), ryv = double(E[1][ty + 1][BX + 1]),
+                         rzv = double(E[2][ty + 1][BX + 1]);
+            {  // Hz at (ir, j)
+                const double d1 = (double(E[1][ty + 1][BX]) - ryv) * inv_dx;
+                const double d2 = (double(E[0][ty][BX + 1]) - rxv) * inv_dx;
+                const T h0 = G[2][ty][BX];
+                H[2][ty][BX] = r_hz && hreal_z(2, kk) ? h_value(S, d1, d2, h0) : h0;
+            }
+            {  // Hy at (ir, j)
+                const double d1 = (double(Ep[0][ty + 1][BX + 1]) - rxv) * inv_dx;
+                const double d2 = (double(E[2][ty + 1][BX]) - rzv) * inv_dx;
+                const T h0 = G[1][ty][BX];
+                H[1][ty][BX] = r_hy && hreal_z(1, kk) ? h_value(S, d1, d2, h0) : h0;
+            }
+        }
+    };
+
+    // ---- prologue: planes kb-1 and kb staged, H^{n+1/2} of plane kb, loads of plane kb+1
+    FPlane<T> PA, PB;
+    load_plane(kb - 1, PA);
+    load_plane(kb, PB);
+    write_planes(kb - 1, PA);
+    write_planes(kb, PB);
+    uint32_t idc = PB.id, sbc[3] = {PB.sb[0], PB.sb[1], PB.sb[2]};
+    __syncthreads();
+    {
+        T h0[3];
+        h_plane(kb, h0);
+    }
+    if (!(kb == ke && above_sh))
+        load_plane(kb + 1, PA);
+
+    // One plane: H^{n+1/2} of plane k+1, then E^{n+1} of plane k.  Pc holds the
+    // loads of plane k+1 (issued one plane earlier); the loads of plane k+2 go to
+    // Pn.  The loop alternates the two sets so that no loaded register is copied.
+    auto plane = [&](int k, const FPlane<T>& Pc, FPlane<T>& Pn) {
+        // the plane above a segment that ends at the box's top is the shell's:
+        // its new H is read, not formed
+        const bool last_sh = k == ke && above_sh;
+        if (!last_sh)
+            write_planes(k + 1, Pc);
+        const uint32_t idn = Pc.id, sbn[3] = {Pc.sb[0], Pc.sb[1], Pc.sb[2]};
+        __syncthreads();
+        // polarization loads of plane k first: waiting for them leaves plane k+2's in flight
+        PlaneIn<T, NPM, CP> L;
+        load_sparse_k(k, idc, sbc, L);
+        if (k + 1 <= ke && !(k + 1 == ke && above_sh))
+            load_plane(k + 2, Pn);
+        T hn[3];
+        if (!last_sh) {
+            h_plane(k + 1, hn);
+        } else {
+            const uint32_t q = off(il, jl, clz(k + 1));
+#pragma unroll
+            for (int C = 0; C < 3; ++C)
+                hn[C] = ldb(S.hout[C], q);
+        }
+        // E^{n+1} of plane k
+        const T(*E)[BY + 2][BX + 2] = es[(k + 3) % 3];
+        const T(*H)[BY + 1][BX + 1] = hs[k & 1];
+        ZCtx Z;
+        Z.k = k;
+        Z.o = X.base + sz * k;
+        Z.sz = -1;
+        Z.bz = Z.cz = 0;
+        Z.ikz = 1;
+#pragma unroll
+        for (int C = 0; C < 3; ++C)
+            L.e[C] = E[C][ty + 1][tx + 1];
+        const double hx = double(H[0][ty][tx]), hy = double(H[1][ty][tx]), hz = double(H[2][ty][tx]);
+        if (X.in[0] && k >= A.lo[0][2] && k <= A.hi[0][2]) {
+            const double d1 = (double(H[2][ty + 1][tx]) - hz) * inv_dx;
+            const double d2 = (double(hn[1]) - hy) * inv_dx;
+            update_comp<T, MAT, NPM, CP, DENSE, 0, false>(A, smat, spc, X, Z, d1, d2, idc, L);
+        }
+        if (X.in[1] && k >= A.lo[1][2] && k <= A.hi[1][2]) {
+            const double d1 = (double(hn[0]) - hx) * inv_dx;
+            const double d2 = (double(H[2][ty][tx + 1]) - hz) * inv_dx;
+            update_comp<T, MAT, NPM, CP, DENSE, 1, false>(A, smat, spc, X, Z, d1, d2, idc, L);
+        }
+        if (X.in[2] && k >= A.lo[2][2] && k <= A.hi[2][2]) {
+            const double d1 = (double(H[1][ty][tx + 1]) - hy) * inv_dx;
+            const double d2 = (double(H[0][ty + 1][tx]) - hx) * inv_dx;
+            update_comp<T, MAT, NPM, CP, DENSE, 2, false>(A, smat, spc, X, Z, d1, d2, idc, L);
+        }
+        idc = idn;
+#pragma unroll
+        for (int C = 0; C < 3; ++C)
+            sbc[C] = sbn[C];
+    };
+    for (int k = kb; k <= ke; k += 2) {
+        plane(k, PA, PB);
+        if (k + 1 <= ke)
+            plane(k + 1, PB, PA);
     }
 }
 
@@ -1123,6 +1557,20 @@ struct EngineImpl {
     int64_t n = 0;  // time_step
 
     void* field[6] = {};
+    // fused full-step mode: field holds E^n / H^{n-1/2}, field_b receives the
+    // step's new values and the two swap after every step; the magnetic CPML
+    // auxiliaries are double-buffered the same way (hq_b: psi1, psi2 per axis)
+    bool fused = false;
+    void* field_b[6] = {};
+    int lzf = 32;
+    uint32_t* ftiles = nullptr;  // fused-kernel tiles: 0 .. nft-1 over its box
+    int nft = 0;
+    int fbox[3][2] = {{0, -1}, {0, -1}, {0, -1}};  // its box of lattice indices
+    int fcount[3] = {0, 0, 0};                       // its tiles per axis
+    // the CPML shell's half-step tiles, per half: clear of that half's slabs, not
+    uint32_t* stiles[2][2] = {};
+    int nst[2][2] = {};
+    double traffic_f[7] = {};
     void* Pa[2] = {nullptr, nullptr};
     void* Pb[2] = {nullptr, nullptr};
     int S_slots[2] = {0, 0};
@@ -1223,6 +1671,21 @@ struct EngineImpl {
     HalfArgs<T> half_args(int h);
     template <typename T>
     void launch_half(int h, hipEvent_t e0, hipEvent_t e1);
+    // sources, E/H DFT monitors and ghost faces of half h after its kernels
+    template <typename T>
+    void post_half(int h);
+    bool fused_eligible(std::string* why) const;
+    template <typename T>
+    StepArgs<T> step_args();
+    template <typename T>
+    void build_fused_tiles();
+    template <typename T>
+    void step_fused(hipEvent_t* ev);
+    template <typename T>
+    void launch_step(size_t lds, const StepArgs<T>& S0);
+    // half-step kernel of half h over a tile list (with or without the CPML paths)
+    template <typename T>
+    void launch_tiles(int h, const HalfArgs<T>& A, const uint32_t* list, int count, bool pml);
     // npm: most poles at any point of the half (register arrays are sized by it);
     // cp: compact polarization layout; U: planes whose loads are issued together
     // Kernel choice.  Electric material kernels with up to three poles are
@@ -1232,7 +1695,7 @@ struct EngineImpl {
     template <typename T, bool IS_E, bool CP, bool PML, bool DENSE>
     void launch_mat(int npm, int U, size_t lds, dim3 grid, dim3 block, const HalfArgs<T>& A)
     {
-#define FFIP_K(NP, UU, P, D) k_half<T, IS_E, true, NP, CP, UU, P, D><<<grid, block, lds, stream>>>(A)
+#define FFIP_K(NP, UU, P, D) k_half<T, IS_E, true, NP, CP, UU, P, D, false><<<grid, block, lds, stream>>>(A)
         // two planes per batch only while the pole registers allow (three poles
         // fit without the CPML paths)
         if (U >= 2 && npm <= (PML ? 2 : 3)) {
@@ -1258,24 +1721,38 @@ struct EngineImpl {
     {
         if (!mat) {
             switch (U) {
-            case 1: k_half<T, IS_E, false, 0, false, 1, PML, false><<<grid, block, 0, stream>>>(A); break;
-            case 2: k_half<T, IS_E, false, 0, false, 2, PML, false><<<grid, block, 0, stream>>>(A); break;
-            default: k_half<T, IS_E, false, 0, false, 4, PML, false><<<grid, block, 0, stream>>>(A); break;
+            case 1: k_half<T, IS_E, false, 0, false, 1, PML, false, false><<<grid, block, 0, stream>>>(A); break;
+            case 2: k_half<T, IS_E, false, 0, false, 2, PML, false, false><<<grid, block, 0, stream>>>(A); break;
+            default: k_half<T, IS_E, false, 0, false, 4, PML, false, false><<<grid, block, 0, stream>>>(A); break;
             }
             return;
         }
         if constexpr (IS_E || (PML && DENSE)) {
             if (npm == 0) {
                 if (U >= 2)
-                    k_half<T, IS_E, true, 0, false, 2, PML, DENSE><<<grid, block, lds, stream>>>(A);
+                    k_half<T, IS_E, true, 0, false, 2, PML, DENSE, false><<<grid, block, lds, stream>>>(A);
                 else
-                    k_half<T, IS_E, true, 0, false, 1, PML, DENSE><<<grid, block, lds, stream>>>(A);
+                    k_half<T, IS_E, true, 0, false, 1, PML, DENSE, false><<<grid, block, lds, stream>>>(A);
             } else if (cp) {
                 launch_mat<T, IS_E, true, PML, DENSE>(npm, U, lds, grid, block, A);
             } else {
                 launch_mat<T, IS_E, false, PML, DENSE>(npm, U, lds, grid, block, A);
             }
         }
+    }
+    // the half-step kernel with the in-kernel (D/B) monitor path: CPML and
+    // per-point-medium paths on, up to MAXS poles, one plane per load batch
+    bool monitors[2] = {false, false};  // some D/B monitor of the half has points
+    template <typename T, bool IS_E>
+    void launch_general(bool mat, size_t lds, dim3 grid, dim3 block, const HalfArgs<T>& A)
+    {
+        const int h = IS_E ? 0 : 1;
+        if (!mat)
+            k_half<T, IS_E, false, 0, false, 1, true, false, true><<<grid, block, 0, stream>>>(A);
+        else if (pcompact[h])
+            k_half<T, IS_E, true, MAXS, true, 1, true, true, true><<<grid, block, lds, stream>>>(A);
+        else
+            k_half<T, IS_E, true, MAXS, false, 1, true, true, true><<<grid, block, lds, stream>>>(A);
     }
     template <typename T, bool IS_E>
     void launch_k(bool mat, int npm, bool cp, int U, bool pml, bool dense, size_t lds, dim3 grid, dim3 block,
@@ -1431,6 +1908,9 @@ void EngineImpl::build()
                                  std::to_string(L.alloc) + " cells)");
     HIP_OK(hipGetDevice(&device));
     HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // zeroed stand-in for absent dense-medium arrays, which the kernels read at
+    // element 0 and ignore (see load_sparse); absent CPML auxiliaries point there too
+    void* stub = dalloc<double>(32);
     for (int c = 0; c < 6; ++c) {
         field[c] = fp32 ? (void*)dalloc<float>(L.alloc) : (void*)dalloc<double>(L.alloc);
     }
@@ -1483,6 +1963,7 @@ void EngineImpl::build()
                 throw std::runtime_error("material table too large for LDS");
             for (int c = 0; c < 3; ++c) {
                 DenseRegion& D = M.dense[c];
+                d_inv[h][c] = static_cast<double*>(stub);
                 if (!D.present)
                     continue;
                 d_inv[h][c] = upload(D.inv_eps);
@@ -1504,6 +1985,7 @@ void EngineImpl::build()
             PmlAxis& P = s.pml[a];
             PmlDev& D = pml[h][a];
             D.W = P.W[par];
+            D.psi1 = D.psi2 = stub;
             if (D.W == 0)
                 continue;
             D.smap = upload(P.smap[par]);
@@ -1562,6 +2044,8 @@ void EngineImpl::build()
         }
         if (units[h].size() > size_t(MAXU))
             throw std::runtime_error("more than 16 DFT units on one half step");
+        for (auto& U : units[h])
+            monitors[h] = monitors[h] || (U.fused && U.npts > 0);
         MF[h] = int(fl.size());
         freqs_dev[h] = upload(fl);
         ph[h] = dalloc<double>(size_t(std::max(1, MF[h])) * 2 * PH_CHUNK);
@@ -1758,6 +2242,58 @@ void EngineImpl::build()
     else
         build_tiles<double>();
     mark("tile lists");
+    // fused full-step kernel: only on request (FFIP_FUSED=1) and where the
+    // simulation allows it.  Measured on MI355X it is slower than the two
+    // half-step kernels (512^3 voxel mix 4.60 vs 4.36 ms per step, 256^3
+    // dispersive 0.471 vs 0.435 ms): it moves a third fewer field bytes but,
+    // holding a plane of loads in registers (3 waves per SIMD), it keeps fewer
+    // bytes in flight.
+    {
+        const char* fe = std::getenv("FFIP_FUSED");
+        fused = fe && std::atoi(fe) != 0 && fused_eligible(nullptr);
+    }
+    if (fused) {
+        for (int c = 0; c < 6; ++c)
+            field_b[c] = fp32 ? (void*)dalloc<float>(L.alloc) : (void*)dalloc<double>(L.alloc);
+        // z segment per workgroup: the longest of 32, 16, 8 planes that still gives
+        // several rounds of workgroups over the CUs (FFIP_LZF overrides)
+        {
+            int cols = 1, nz = 0;
+            for (int a = 0; a < 2; ++a) {
+                int lo = 1 << 30, hi = -1;
+                for (int c = 0; c < 6; ++c)
+                    if (L.hi[c][a] >= L.lo[c][a]) {
+                        lo = std::min(lo, L.lo[c][a]);
+                        hi = std::max(hi, L.hi[c][a]);
+                    }
+                cols *= (hi - lo + (a == 0 ? BX : BY)) / (a == 0 ? BX : BY);
+            }
+            {
+                int lo = 1 << 30, hi = -1;
+                for (int c = 0; c < 6; ++c)
+                    if (L.hi[c][2] >= L.lo[c][2]) {
+                        lo = std::min(lo, L.lo[c][2]);
+                        hi = std::max(hi, L.hi[c][2]);
+                    }
+                nz = hi - lo + 1;
+            }
+            // a multiple of the half-step kernels' segment, so that their shell
+            // tiles never straddle a fused segment
+            lzf = lz;
+            for (int c : {32, 16, 8})
+                if (c % lz == 0 && int64_t(cols) * ((nz + c - 1) / c) >= 4096) {
+                    lzf = c;
+                    break;
+                }
+            if (const char* e = std::getenv("FFIP_LZF"))
+                lzf = std::max(lz, std::atoi(e) / lz * lz);
+        }
+        if (fp32)
+            build_fused_tiles<float>();
+        else
+            build_fused_tiles<double>();
+        mark("fused step");
+    }
     HIP_OK(hipStreamSynchronize(stream));
     mark("copies and fills");
 }
@@ -1770,7 +2306,8 @@ HalfArgs<T> EngineImpl::half_args(int h)
     HalfArgs<T> A;
     std::memset(&A, 0, sizeof(A));
     for (int c = 0; c < 3; ++c) {
-        A.u[c] = static_cast<T*>(field[h * 3 + c]);
+        A.u[c] = static_cast<const T*>(field[h * 3 + c]);
+        A.w[c] = static_cast<T*>(field[h * 3 + c]);
         A.s[c] = static_cast<const T*>(field[(1 - h) * 3 + c]);
     }
     A.pitch = int(L.pitch);
@@ -1807,6 +2344,7 @@ HalfArgs<T> EngineImpl::half_args(int h)
         A.npoles = npoles[h];
         for (int c = 0; c < 3; ++c) {
             const DenseRegion& D = M.dense[c];
+            A.d_inv[c] = d_inv[h][c];  // the zeroed stand-in when absent
             if (!D.present)
                 continue;
             for (int a = 0; a < 3; ++a) {
@@ -1935,40 +2473,61 @@ void EngineImpl::build_tiles()
 }
 
 template <typename T>
+void EngineImpl::launch_tiles(int h, const HalfArgs<T>& A, const uint32_t* list, int count, bool pml)
+{
+    if (count == 0)
+        return;
+    const HalfMaterial& M = S->mat[h];
+    const bool mat = !M.uniform;
+    const size_t lds = mat ? size_t(A.nmats + A.npoles) * 3 * sizeof(double) : 0;
+    // pole kernels on the CPML tiles of a split half step take unroll_shell planes per batch
+    const bool split = fused || !pml_a[h];
+    const int u = pml && split && unroll_shell > 0 && mat && S_slots[h] > 0
+                      ? unroll_shell
+                      : (mat && S_slots[h] > 0 ? unroll_mat : unroll);
+    HalfArgs<T> X = A;
+    X.tiles = list;
+    X.ntiles = count;
+    X.nblocks = ((count + 7) / 8) * 8;
+    const dim3 grid(X.nblocks, 1, 1), block(BX, BY, 1);
+    if (monitors[h]) {
+        // flux-density monitors accumulate inside the kernel: the general form
+        if (h == 0)
+            launch_general<T, true>(mat, lds, grid, block, X);
+        else
+            launch_general<T, false>(mat, lds, grid, block, X);
+    } else if (h == 0) {
+        launch_k<T, true>(mat, S_slots[h], pcompact[h], u, pml, dense[h], lds, grid, block, X);
+    } else {
+        launch_k<T, false>(mat, S_slots[h], pcompact[h], u, pml, dense[h], lds, grid, block, X);
+    }
+    HIP_OK(hipGetLastError());
+}
+
+template <typename T>
 void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
 {
     HalfArgs<T> A = half_args<T>(h);
     if (A.dhi[0] < A.dlo[0] || A.dhi[1] < A.dlo[1] || A.dhi[2] < A.dlo[2])
         return;
-    dim3 block(BX, BY, 1);
-    const HalfMaterial& M = S->mat[h];
-    const bool mat = !M.uniform;
-    const int U = mat && S_slots[h] > 0 ? unroll_mat : unroll;
-    const size_t lds = mat ? size_t(A.nmats + A.npoles) * 3 * sizeof(double) : 0;
     // tiles clear of the CPML without its paths (when split), then the others
-    auto go = [&](const uint32_t* list, int count, bool pml) {
-        if (count == 0)
-            return;
-        const int u = pml && !pml_a[h] && unroll_shell > 0 && mat && S_slots[h] > 0 ? unroll_shell : U;
-        HalfArgs<T> X = A;
-        X.tiles = list;
-        X.ntiles = count;
-        X.nblocks = ((count + 7) / 8) * 8;
-        dim3 grid(X.nblocks, 1, 1);
-        if (h == 0)
-            launch_k<T, true>(mat, S_slots[h], pcompact[h], u, pml, dense[h], lds, grid, block, X);
-        else
-            launch_k<T, false>(mat, S_slots[h], pcompact[h], u, pml, dense[h], lds, grid, block, X);
-        HIP_OK(hipGetLastError());
-    };
     if (e0)
         HIP_OK(hipEventRecord(e0, stream));
-    go(tiles_a[h], ntiles_a[h], pml_a[h]);
-    go(tiles_b[h], ntiles_b[h], true);
+    launch_tiles<T>(h, A, tiles_a[h], ntiles_a[h], pml_a[h]);
+    launch_tiles<T>(h, A, tiles_b[h], ntiles_b[h], true);
     if (e1)
         HIP_OK(hipEventRecord(e1, stream));
     if (S_slots[h] > 0)
         std::swap(Pa[h], Pb[h]);
+    post_half<T>(h);
+}
+
+// Point sources, E/H DFT monitors and ghost faces of half h, after its kernels
+// (they read and write the current field buffers)
+template <typename T>
+void EngineImpl::post_half(int h)
+{
+    const double* phr = ph[h] + size_t(n - ph_start) * 2 * std::max(1, MF[h]);
     // sources of this half at its source time (src/simulation.cpp:748-766)
     const double t = h == 1 ? double(n) * S->dt : (double(n) + 0.5) * S->dt;
     Src& SR = src[h];
@@ -1992,7 +2551,7 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
         B.dt = S->dt;
         for (size_t u = 0; u < units[h].size(); ++u)
             B.units[u] = units[h][u];
-        B.ph = A.ph;
+        B.ph = phr;
         k_sources<T><<<(SR.ntarget + 255) / 256, 256, 0, stream>>>(B, t);
         HIP_OK(hipGetLastError());
     }
@@ -2011,7 +2570,7 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
                 most = std::max(most, U.npts);
                 single = single && U.nf == 1;
             }
-        D.ph = A.ph;
+        D.ph = phr;
         if (D.nunits > 0) {
             constexpr int PT = 2;  // measured: 2 and 1 beat 4 (design monitors 4.2 vs 4.7 us FP32)
             // FP32 engines keep partial sums, added into the FP64 sums every DFT_FLUSH steps
@@ -2064,12 +2623,288 @@ void EngineImpl::launch_half(int h, hipEvent_t e0, hipEvent_t e1)
     }
 }
 
+// Whether the fused full-step kernel can run this simulation: the magnetic half
+// must be one pole-free medium with no point sources, flux-density monitors or
+// ghost faces (the kernel forms magnetic values of neighbouring tiles itself and
+// runs no magnetic source or face pass between the halves).
+bool EngineImpl::fused_eligible(std::string* why) const
+{
+    auto no = [&](const char* w) {
+        if (why)
+            *why = w;
+        return false;
+    };
+    const Setup& s = *S;
+    if (!s.mat[1].uniform)
+        return no("magnetic media");
+    if (src[1].ntarget > 0)
+        return no("magnetic sources");
+    if (!faces[0].empty() || !faces[1].empty())
+        return no("periodic or symmetry faces");
+    if (monitors[0] || monitors[1])
+        return no("flux-density monitors");
+    if (S_slots[0] > 3)
+        return no("more than three poles at a point");
+    if (dense[0] && S_slots[0] > 0)
+        return no("per-point media with poles");
+    const Layout& L = s.L;
+    for (int a = 0; a < 3; ++a) {
+        int lo[2] = {1 << 30, 1 << 30}, hi[2] = {-1, -1};
+        for (int h = 0; h < 2; ++h)
+            for (int c = 0; c < 3; ++c)
+                if (L.hi[h * 3 + c][a] >= L.lo[h * 3 + c][a]) {
+                    lo[h] = std::min(lo[h], L.lo[h * 3 + c][a]);
+                    hi[h] = std::max(hi[h], L.hi[h * 3 + c][a]);
+                }
+        if (lo[0] != lo[1] || hi[0] != hi[1] || hi[0] < lo[0])
+            return no("electric and magnetic lattices cover different ranges");
+    }
+    return true;
+}
+
+template <typename T>
+StepArgs<T> EngineImpl::step_args()
+{
+    StepArgs<T> S0;
+    std::memset(&S0, 0, sizeof(S0));
+    S0.e = half_args<T>(0);
+    HalfArgs<T>& A = S0.e;
+    for (int c = 0; c < 3; ++c) {
+        A.w[c] = static_cast<T*>(field_b[c]);
+        A.s[c] = nullptr;
+        S0.hin[c] = static_cast<const T*>(field[3 + c]);
+        S0.hout[c] = static_cast<T*>(field_b[3 + c]);
+        for (int a = 0; a < 3; ++a) {
+            S0.hlo[c][a] = S->L.lo[3 + c][a];
+            S0.hhi[c][a] = S->L.hi[3 + c][a];
+        }
+    }
+    S0.h_inv = S->mat[1].u_inv;
+    A.lz = lzf;
+    for (int a = 0; a < 3; ++a) {
+        S0.fo[a] = fbox[a][0];
+        S0.fe[a] = fbox[a][1];
+        S0.fn[a] = fcount[a];
+    }
+    return S0;
+}
+
+// The fused kernel's box: per axis, the longest run of whole tiles (BX, BY, lz
+// points, counted from the half step's first index) whose electric and magnetic
+// points all lie clear of the CPML slabs.  The fused kernel tiles it with
+// segments of lzf planes along z; the half-step kernels cover the rest of the
+// grid (the CPML shell) with their own tiles, listed per half as clear of that
+// half's slabs (stiles[h][0]) or not (stiles[h][1]).
+template <typename T>
+void EngineImpl::build_fused_tiles()
+{
+    const Setup& s = *S;
+    const Layout& L = s.L;
+    const HalfArgs<T> A = half_args<T>(0);
+    const int step[3] = {BX, BY, A.lz};
+    for (int a = 0; a < 3; ++a) {
+        const PmlAxis& P = s.pml[a];
+        auto clear = [&](int x) {
+            return (P.W[0] == 0 || P.smap[0][x] < 0) && (P.W[1] == 0 || P.smap[1][x] < 0);
+        };
+        // longest run of clear indices
+        int best0 = 0, best1 = -1;
+        for (int x = A.dlo[a]; x <= A.dhi[a];) {
+            if (!clear(x)) {
+                ++x;
+                continue;
+            }
+            int y = x;
+            while (y + 1 <= A.dhi[a] && clear(y + 1))
+                ++y;
+            if (y - x > best1 - best0) {
+                best0 = x;
+                best1 = y;
+            }
+            x = y + 1;
+        }
+        // round inwards to whole tiles (a run that reaches the grid's end keeps it)
+        const int first = A.dlo[a] + (best0 - A.dlo[a] + step[a] - 1) / step[a] * step[a];
+        const int last = best1 == A.dhi[a] ? A.dhi[a] : A.dlo[a] + (best1 + 1 - A.dlo[a]) / step[a] * step[a] - 1;
+        fbox[a][0] = first;
+        fbox[a][1] = last;
+    }
+    const bool empty = fbox[0][1] < fbox[0][0] || fbox[1][1] < fbox[1][0] || fbox[2][1] < fbox[2][0];
+    for (int a = 0; a < 3; ++a)
+        fcount[a] = empty ? 0 : (fbox[a][1] - fbox[a][0] + (a == 0 ? BX : a == 1 ? BY : lzf)) / (a == 0 ? BX : a == 1 ? BY : lzf);
+    std::vector<uint32_t> in;
+    for (int q = 0; q < fcount[0] * fcount[1] * fcount[2]; ++q)
+        in.push_back(uint32_t(q));
+    nft = int(in.size());
+    ftiles = upload(in);
+    auto inbox = [&](int a, int lo, int hi) { return !empty && lo >= fbox[a][0] && hi <= fbox[a][1]; };
+    for (int h = 0; h < 2; ++h) {
+        HalfArgs<T> B = half_args<T>(h);
+        const int par = h == 0 ? 0 : 1, bstep[3] = {BX, BY, B.lz}, bn[3] = {B.gx, B.gy, B.gz};
+        std::vector<char> touch[3], inside[3];
+        for (int a = 0; a < 3; ++a) {
+            touch[a].assign(size_t(bn[a]), 0);
+            inside[a].assign(size_t(bn[a]), 0);
+            const PmlAxis& P = s.pml[a];
+            for (int q = 0; q < bn[a]; ++q) {
+                const int lo = B.dlo[a] + q * bstep[a], hi = std::min(lo + bstep[a] - 1, B.dhi[a]);
+                inside[a][q] = inbox(a, lo, hi);
+                for (int x = lo; x <= hi && P.W[par]; ++x)
+                    if (P.smap[par][x] >= 0)
+                        touch[a][q] = 1;
+            }
+        }
+        std::vector<uint32_t> la, lb;
+        for (int tz = 0; tz < B.gz; ++tz)
+            for (int ty = 0; ty < B.gy; ++ty)
+                for (int tx = 0; tx < B.gx; ++tx) {
+                    if (inside[0][tx] && inside[1][ty] && inside[2][tz])
+                        continue;  // the fused kernel's
+                    const uint32_t t = uint32_t(tx + B.gx * (ty + B.gy * tz));
+                    (touch[0][tx] || touch[1][ty] || touch[2][tz] ? lb : la).push_back(t);
+                }
+        nst[h][0] = int(la.size());
+        nst[h][1] = int(lb.size());
+        stiles[h][0] = upload(la);
+        stiles[h][1] = upload(lb);
+    }
+    // algorithmic bytes of one fused-kernel launch: every field element of its
+    // points read and written once, and the electric half's material words,
+    // polarization state and flux-density monitor sums there
+    auto cnt = [&](int comp, int a, int lo, int hi) {
+        if (empty)
+            return int64_t(0);
+        const int l = std::max({lo, L.lo[comp][a], fbox[a][0]}), u = std::min({hi, L.hi[comp][a], fbox[a][1]});
+        return int64_t(std::max(0, u - l + 1));
+    };
+    const int big = 1 << 30;
+    double pts = 0, mcells = 0, ade = 0, dftb = 0;
+    for (int c = 0; c < 6; ++c)
+        pts += double(cnt(c, 0, -big, big)) * double(cnt(c, 1, -big, big)) * double(cnt(c, 2, -big, big));
+    const HalfMaterial& M = s.mat[0];
+    if (!M.uniform && !empty) {
+        mcells = 1;
+        for (int a = 0; a < 3; ++a)
+            mcells *= double(std::max(0, std::min(M.mbox[a][1], fbox[a][1]) - std::max(M.mbox[a][0], fbox[a][0]) + 1));
+        std::vector<double> per(size_t(L.n[2]), 0.0);
+        parallel_planes(fbox[2][0], fbox[2][1], [&](int k) {
+            double acc = 0;
+            for (int c = 0; c < 3; ++c) {
+                if (k < L.lo[c][2] || k > L.hi[c][2])
+                    continue;
+                for (int j = std::max(L.lo[c][1], fbox[1][0]); j <= std::min(L.hi[c][1], fbox[1][1]); ++j)
+                    for (int i = std::max(L.lo[c][0], fbox[0][0]); i <= std::min(L.hi[c][0], fbox[0][1]); ++i) {
+                        const int id = M.ids[c][L.index(i, j, k)];
+                        acc += double(id == 255 ? M.dense[c].npoles : M.table.npoles[id]);
+                    }
+            }
+            per[size_t(k)] = acc;
+        });
+        for (double v : per)
+            ade += 3.0 * v;  // read P, read P-, write P+
+    }
+    for (auto& U : units[0])
+        if (U.fused && U.npts > 0)
+            dftb += double(cnt(U.comp, 0, U.box[0][0], U.box[0][1])) * double(cnt(U.comp, 1, U.box[1][0], U.box[1][1])) *
+                    double(cnt(U.comp, 2, U.box[2][0], U.box[2][1])) * U.nf * 32.0;
+    traffic_f[0] = 2 * pts * double(tsz);
+    traffic_f[1] = 4.0 * mcells;
+    traffic_f[2] = ade * double(tsz);
+    traffic_f[3] = 0;
+    traffic_f[4] = dftb;
+    traffic_f[5] = traffic_f[0] + traffic_f[1] + traffic_f[2] + traffic_f[3] + traffic_f[4];
+    traffic_f[6] = pts;
+}
+
+template <typename T>
+void EngineImpl::launch_step(size_t lds, const StepArgs<T>& S0)
+{
+    if (nft == 0)
+        return;
+    StepArgs<T> X = S0;
+    X.e.tiles = ftiles;
+    X.e.ntiles = nft;
+    X.e.nblocks = ((nft + 7) / 8) * 8;
+    const dim3 grid(X.e.nblocks, 1, 1), block(BX, BY, 1);
+    const bool mat = !S->mat[0].uniform;
+    const int npm = S_slots[0];
+    if (!mat) {
+        k_step<T, false, 0, false, false><<<grid, block, 0, stream>>>(X);
+    } else if (dense[0]) {
+        k_step<T, true, 0, false, true><<<grid, block, lds, stream>>>(X);
+    } else if (npm == 0) {
+        k_step<T, true, 0, false, false><<<grid, block, lds, stream>>>(X);
+    } else if (pcompact[0]) {
+        switch (npm) {
+        case 1: k_step<T, true, 1, true, false><<<grid, block, lds, stream>>>(X); break;
+        case 2: k_step<T, true, 2, true, false><<<grid, block, lds, stream>>>(X); break;
+        default: k_step<T, true, 3, true, false><<<grid, block, lds, stream>>>(X); break;
+        }
+    } else {
+        switch (npm) {
+        case 1: k_step<T, true, 1, false, false><<<grid, block, lds, stream>>>(X); break;
+        case 2: k_step<T, true, 2, false, false><<<grid, block, lds, stream>>>(X); break;
+        default: k_step<T, true, 3, false, false><<<grid, block, lds, stream>>>(X); break;
+        }
+    }
+    HIP_OK(hipGetLastError());
+}
+
+// One time step in fused mode, from the current buffers into the other ones:
+// the magnetic half-step kernel on the CPML shell, the fused kernel on the
+// interior, the electric half-step kernel on the shell (it reads the new H of
+// both); then the buffers swap, and the electric sources and the monitors
+// follow as after the two half-step kernels.
+// ev (may be null): ev[0], ev[1] bracket the fused kernel, ev[2], ev[3] the whole
+// step's kernels
+template <typename T>
+void EngineImpl::step_fused(hipEvent_t* ev)
+{
+    StepArgs<T> S0 = step_args<T>();
+    const size_t lds = S->mat[0].uniform ? 0 : size_t(S0.e.nmats + S0.e.npoles) * 3 * sizeof(double);
+    if (ev)
+        HIP_OK(hipEventRecord(ev[2], stream));
+    {
+        HalfArgs<T> A = half_args<T>(1);
+        for (int c = 0; c < 3; ++c)
+            A.w[c] = static_cast<T*>(field_b[3 + c]);
+        launch_tiles<T>(1, A, stiles[1][0], nst[1][0], false);
+        launch_tiles<T>(1, A, stiles[1][1], nst[1][1], true);
+    }
+    if (ev)
+        HIP_OK(hipEventRecord(ev[0], stream));
+    launch_step<T>(lds, S0);
+    if (ev)
+        HIP_OK(hipEventRecord(ev[1], stream));
+    {
+        HalfArgs<T> A = half_args<T>(0);
+        for (int c = 0; c < 3; ++c) {
+            A.w[c] = static_cast<T*>(field_b[c]);
+            A.s[c] = static_cast<const T*>(field_b[3 + c]);
+        }
+        launch_tiles<T>(0, A, stiles[0][0], nst[0][0], false);
+        launch_tiles<T>(0, A, stiles[0][1], nst[0][1], true);
+    }
+    if (ev)
+        HIP_OK(hipEventRecord(ev[3], stream));
+    for (int c = 0; c < 6; ++c)
+        std::swap(field[c], field_b[c]);
+    if (S_slots[0] > 0)
+        std::swap(Pa[0], Pb[0]);
+    post_half<T>(1);
+    post_half<T>(0);
+}
+
 template <typename T>
 void EngineImpl::step_once(hipEvent_t* ev, bool timed_e, bool timed_h)
 {
     ensure_phasors();
-    launch_half<T>(1, timed_h ? ev[2] : nullptr, timed_h ? ev[3] : nullptr);
-    launch_half<T>(0, timed_e ? ev[0] : nullptr, timed_e ? ev[1] : nullptr);
+    if (fused) {
+        step_fused<T>(timed_e || timed_h ? ev : nullptr);
+    } else {
+        launch_half<T>(1, timed_h ? ev[2] : nullptr, timed_h ? ev[3] : nullptr);
+        launch_half<T>(0, timed_e ? ev[0] : nullptr, timed_e ? ev[1] : nullptr);
+    }
     if (nprobes && n < trace_cap) {
         ProbeArgs<T> P;
         std::memset(&P, 0, sizeof(P));
@@ -2130,11 +2965,15 @@ void Engine::sync() { HIP_OK(hipStreamSynchronize(p_->stream)); }
 
 void Engine::traffic(int h, double out[7]) const
 {
-    if (h < 0 || h > 1)
-        throw std::runtime_error("half must be 0 (electric) or 1 (magnetic)");
+    if (h < 0 || h > 2)
+        throw std::runtime_error("half must be 0 (electric), 1 (magnetic) or 2 (fused step)");
+    if (h == 2 && !p_->fused)
+        throw std::runtime_error("this simulation does not run the fused step kernel");
     for (int q = 0; q < 7; ++q)
-        out[q] = p_->traffic[h][q];
+        out[q] = h == 2 ? p_->traffic_f[q] : p_->traffic[h][q];
 }
+
+bool Engine::fused() const { return p_->fused; }
 double Engine::device_bytes() const { return p_->bytes; }
 
 void Engine::step(int64_t nsteps)
@@ -2187,8 +3026,9 @@ KernelTiming Engine::step_timed(int64_t nsteps, bool per_kernel)
             float a = 0, b = 0;
             HIP_OK(hipEventElapsedTime(&a, ev[size_t(q) * 4 + 0], ev[size_t(q) * 4 + 1]));
             HIP_OK(hipEventElapsedTime(&b, ev[size_t(q) * 4 + 2], ev[size_t(q) * 4 + 3]));
+            // fused mode: a = the fused kernel, b = all kernels of the step
             r.e_ms += a;
-            r.h_ms += b;
+            r.h_ms += p_->fused ? b - a : b;
         }
         for (auto& e : ev)
             (void)hipEventDestroy(e);

@@ -83,6 +83,8 @@ public:
     // total, points updated}.  Each array element the half step must read or write
     // counts once; neighbour re-reads and table lookups count zero.
     void traffic(int h, double out[7]) const;
+    // whether each step runs as one fused kernel (h = 2 in traffic())
+    bool fused() const;
 
 private:
     std::unique_ptr<EngineImpl> p_;

@@ -93,7 +93,10 @@ int ffip_sim_get_geom_info(const ffip_sim *sim, int k, double *x, double *y, dou
 
 /* Step nsteps with HIP events on the handle's stream: out[0] = elapsed ms of the
  * whole region, out[1] / out[2] = summed ms of the electric / magnetic half-step
- * kernels (only when per_kernel != 0).  No reference counterpart: measurement. */
+ * kernels (only when per_kernel != 0).  When the simulation runs the fused
+ * full-step kernel (ffip_sim_traffic half 2 succeeds), out[1] is that kernel's
+ * time and out[2] the time of the half-step kernels that cover the CPML shell.
+ * No reference counterpart: measurement. */
 int ffip_sim_step_timed(ffip_sim *sim, int64_t nsteps, int per_kernel, double out[3]);
 
 /* Host-only dry run of the set-up (no GPU needed): writes a JSON summary of the
@@ -102,10 +105,12 @@ int ffip_sim_step_timed(ffip_sim *sim, int64_t nsteps, int per_kernel, double ou
  * call are included.  No reference counterpart: diagnostics and tests. */
 int ffip_sim_plan(const ffip_sim *sim, char *buf, size_t cap, size_t *needed);
 
-/* Algorithmic HBM bytes of one launch of the half-step kernel (half 0 =
- * electric, 1 = magnetic): out = {fields, material ids, polarization state,
- * CPML auxiliaries, DFT sums, total, points updated}.  No reference counterpart:
- * measurement. */
+/* Algorithmic HBM bytes of one launch of the half-step kernel over the whole
+ * grid (half 0 = electric, 1 = magnetic) or of the fused full-step kernel over
+ * the grid's interior clear of the CPML (half 2; FFIP_EINVAL when the
+ * simulation does not run it): out = {fields, material ids, polarization
+ * state, CPML auxiliaries, DFT sums, total, points updated}.  No reference
+ * counterpart: measurement. */
 int ffip_sim_traffic(const ffip_sim *sim, int half, double out[7]);
 
 /* Grid facts: out[0..2] = Yee cells per axis, out[3] = device bytes held,
