@@ -8,9 +8,10 @@
 //                         polarization sums formed in the update)
 // Point sources (Gaussian_Dipoles_Stepping::step, src/fields.cpp:17-33) touch a
 // few points only and run in a small follow-up kernel that also patches the DFT
-// sums of the points it changes; E/H DFT monitors are accumulated by a
-// one-thread-per-point kernel after it; the ghost fills of Period/PEC/PMC
-// boundaries (Fields::sync_boundary, src/fields.cpp:327-482) come last.
+// sums of the points it changes; the ghost fills of Period/PEC/PMC boundaries
+// (Fields::sync_boundary, src/fields.cpp:327-482) follow.  E/H DFT monitors
+// gather each step's samples into a ring, added into their FP64 sums in
+// batches (k_dft_gather, k_dft_ring).
 //
 // The update is written in "E form": E^{n+1} = E^n + (dt*curl - (sum P^{n+1} -
 // sum P^n)) / eps_inf.  Algebraically this is the reference's D form
@@ -68,18 +69,19 @@ struct DftDev {
     int box[3][2];
     int dims[3];
     int nf, mf0, is_db;
-    int fused;  // accumulated inside the half-step kernel (else by k_dft_sparse)
+    int fused;  // accumulated inside the half-step kernel (else through the sample ring)
     int64_t npts;
     double* re;
     double* im;
-    // FP32 storage, units summed by k_dft_sparse*: FP32 partial sums of the last
-    // few steps, added into re/im every DFT_FLUSH steps and before any read
-    float* pre;
-    float* pim;
+    // E/H units: the stored samples of the last steps, [DFT_BATCH][npts] in the
+    // field's storage type, added into re/im by k_dft_ring every DFT_BATCH steps
+    // and before any read
+    void* ring;
 };
 
-// steps between flushes of the FP32 partial DFT sums into the FP64 sums
-constexpr int DFT_FLUSH = 16;
+// steps whose E/H monitor samples are gathered before they are added into the
+// FP64 sums (one read-modify-write of the sums per batch instead of per step)
+constexpr int DFT_BATCH = 16;
 
 struct PmlDev {
     const int* smap;
@@ -1080,155 +1082,84 @@ __global__ void __launch_bounds__(BX * BY) k_step(const StepArgs<T> S)
     }
 }
 
-// DFT units that are too narrow along x to share the half-step kernel's
-// coalesced rows (planes normal to x): one thread per point, run after the
-// point sources so they read final values.
+// E/H DFT monitors (DFT_Unit::update, src/dft.cpp:29-62).  Every step, after the
+// point sources, k_dft_gather copies each monitored point's stored value into
+// its unit's sample ring; every DFT_BATCH steps (and before any read of the
+// sums) k_dft_ring adds the batch into the FP64 running sums:
+//   re[f][p] += cos(w_f t_s) x_s[p],  im[f][p] += -sin(w_f t_s) x_s[p]
+// for s in step order, one FP64 multiply-add per term exactly as a per-step
+// accumulation would do it.  The ring holds the stored (storage-precision)
+// values, which are the values the per-step form would read, so FP32 storage
+// keeps FP64 sums.  Per point and frequency the sums are read and written once
+// per batch: 32 B per DFT_BATCH steps instead of 32 B per step.
 template <typename T>
-struct SparseDftArgs {
-    const T* u[3];
+struct RingArgs {
+    const T* u[6];  // E then H components (offset to lattice index 0)
     int pitch, plane;
     int nunits;
-    DftDev units[MAXU];
-    const double* ph;
-    int npb, nch;  // k_dft_sparse: 256-point blocks and frequency chunks per unit
+    int64_t first[MAXU * 2 + 1];  // unit q's points are gather threads first[q] .. first[q+1]-1
+    DftDev units[MAXU * 2];
+    int half[MAXU * 2];           // 0: electric unit, 1: magnetic
+    int slot;                     // ring slot of this step
+    int cnt;                      // k_dft_ring: samples in the ring
+    const double* ph[2];          // k_dft_ring: phasor rows of the batch's first step, per half
+    int mf[2];                    // phasor row length (frequencies) per half
 };
 
-// MODE 0: FP64 sums; 1: FP32 partial sums; 2: partial sums added into the FP64
-// sums (and cleared) together with this step's sample
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_dft_sparse(const SparseDftArgs<T> A)
+template <typename T>
+__global__ void __launch_bounds__(256) k_dft_gather(const RingArgs<T> A)
 {
-    const DftDev& U = A.units[blockIdx.y];
-    // frequencies in chunks of FC: all sums of a chunk are read before any is
-    // written.  The chunks of one point block are consecutive slots of one XCD
-    // (workgroups go to the 8 XCDs in turn), so the field words they share,
-    // one cache line per point on a plane normal to x, come from that XCD's L2.
-    constexpr int FC = 4;
-    const int b = int(blockIdx.x);
-    const int slot = (b & 7) * int(gridDim.x >> 3) + (b >> 3);
-    const int pb = slot / A.nch;
-    const int f0 = (slot - pb * A.nch) * FC;
-    const int t = pb * int(blockDim.x) + int(threadIdx.x);  // unit sizes are checked < 2^31
-    if (pb >= A.npb || t >= int(U.npts) || f0 >= U.nf)
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= A.first[A.nunits])
         return;
-    const int ii = t % U.dims[0];
-    const int r = t / U.dims[0];
+    int q = 0;
+    while (t >= A.first[q + 1])
+        ++q;
+    const DftDev& U = A.units[q];
+    const int p = int(t - A.first[q]);  // unit sizes are checked < 2^31
+    const int ii = p % U.dims[0];
+    const int r = p / U.dims[0];
     const int jj = r % U.dims[1];
     const int kk = r / U.dims[1];
     const int o = (U.box[0][0] + ii) + A.pitch * (U.box[1][0] + jj) + A.plane * (U.box[2][0] + kk);
-    const double v = double(A.u[U.comp][o]);
-    const double* ph = A.ph + 2 * U.mf0;
-    double* __restrict__ re = U.re + t;
-    double* __restrict__ im = U.im + t;
-    const int64_t stride = U.npts;
-    if constexpr (MODE == 0) {
-        double a[FC], b[FC];
-#pragma unroll
-        for (int q = 0; q < FC; ++q)
-            if (f0 + q < U.nf) {
-                a[q] = re[(f0 + q) * stride];
-                b[q] = im[(f0 + q) * stride];
-            }
-#pragma unroll
-        for (int q = 0; q < FC; ++q)
-            if (f0 + q < U.nf) {
-                re[(f0 + q) * stride] = a[q] + ph[2 * (f0 + q)] * v;
-                im[(f0 + q) * stride] = b[q] + ph[2 * (f0 + q) + 1] * v;
-            }
-    } else {
-        float* __restrict__ pr = U.pre + t;
-        float* __restrict__ pi = U.pim + t;
-        float a[FC], b[FC];
-        double ra[FC], rb[FC];
-#pragma unroll
-        for (int q = 0; q < FC; ++q)
-            if (f0 + q < U.nf) {
-                a[q] = pr[(f0 + q) * stride];
-                b[q] = pi[(f0 + q) * stride];
-                if constexpr (MODE == 2) {
-                    ra[q] = re[(f0 + q) * stride];
-                    rb[q] = im[(f0 + q) * stride];
-                }
-            }
-#pragma unroll
-        for (int q = 0; q < FC; ++q)
-            if (f0 + q < U.nf) {
-                const double x = double(a[q]) + ph[2 * (f0 + q)] * v;
-                const double y = double(b[q]) + ph[2 * (f0 + q) + 1] * v;
-                if constexpr (MODE == 1) {
-                    pr[(f0 + q) * stride] = float(x);
-                    pi[(f0 + q) * stride] = float(y);
-                } else {
-                    re[(f0 + q) * stride] = ra[q] + x;
-                    im[(f0 + q) * stride] = rb[q] + y;
-                    pr[(f0 + q) * stride] = 0.f;
-                    pi[(f0 + q) * stride] = 0.f;
-                }
-            }
-    }
+    const T* u = A.u[A.half[q] * 3 + U.comp];
+    static_cast<T*>(U.ring)[int64_t(A.slot) * U.npts + p] = u[o];
 }
 
-// Single-frequency units (the adjoint loop's monitors): PT points per thread,
-// strided by the grid, with every load of a thread issued before any update.
-template <typename T, int PT, int MODE>
-__global__ void __launch_bounds__(256) k_dft_sparse1(const SparseDftArgs<T> A)
+// one thread per point of unit blockIdx.y: the batch's samples in registers, then
+// every frequency's sums; the unit's phasors of the batch are staged in LDS
+template <typename T>
+__global__ void __launch_bounds__(256) k_dft_ring(const RingArgs<T> A)
 {
+    extern __shared__ double rph[];  // [cnt][nf][2]
     const DftDev& U = A.units[blockIdx.y];
-    const int n = int(U.npts);
-    const int stride = int(gridDim.x * blockDim.x);
-    const int t0 = int(blockIdx.x * blockDim.x + threadIdx.x);
-    const T* u = A.u[U.comp];
-    const double c = A.ph[2 * U.mf0], s = A.ph[2 * U.mf0 + 1];
-    double v[PT], a[PT], b[PT];
-    float fa[PT], fb[PT];
-#pragma unroll
-    for (int p = 0; p < PT; ++p) {
-        const int t = t0 + p * stride;
-        if (t < n) {
-            const int ii = t % U.dims[0];
-            const int r = t / U.dims[0];
-            const int jj = r % U.dims[1];
-            const int kk = r / U.dims[1];
-            const int o = (U.box[0][0] + ii) + A.pitch * (U.box[1][0] + jj) + A.plane * (U.box[2][0] + kk);
-            v[p] = double(u[o]);
-            if constexpr (MODE != 1) {
-                a[p] = U.re[t];
-                b[p] = U.im[t];
-            }
-            if constexpr (MODE != 0) {
-                fa[p] = U.pre[t];
-                fb[p] = U.pim[t];
-            }
-        }
+    const int h = A.half[blockIdx.y];
+    const int nf = U.nf, cnt = A.cnt;
+    for (int q = int(threadIdx.x); q < cnt * nf; q += int(blockDim.x)) {
+        const int st = q / nf, f = q - st * nf;
+        const double* row = A.ph[h] + int64_t(st) * 2 * A.mf[h] + 2 * (U.mf0 + f);
+        rph[2 * q] = row[0];
+        rph[2 * q + 1] = row[1];
     }
+    __syncthreads();
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= U.npts)
+        return;
+    const T* ring = static_cast<const T*>(U.ring);
+    double v[DFT_BATCH];
 #pragma unroll
-    for (int p = 0; p < PT; ++p) {
-        const int t = t0 + p * stride;
-        if (t < n) {
-            if constexpr (MODE == 0) {
-                U.re[t] = a[p] + c * v[p];
-                U.im[t] = b[p] + s * v[p];
-            } else if constexpr (MODE == 1) {
-                U.pre[t] = float(double(fa[p]) + c * v[p]);
-                U.pim[t] = float(double(fb[p]) + s * v[p]);
-            } else {
-                U.re[t] = a[p] + (double(fa[p]) + c * v[p]);
-                U.im[t] = b[p] + (double(fb[p]) + s * v[p]);
-                U.pre[t] = 0.f;
-                U.pim[t] = 0.f;
+    for (int st = 0; st < DFT_BATCH; ++st)
+        v[st] = st < cnt ? double(ring[int64_t(st) * U.npts + p]) : 0.0;
+    for (int f = 0; f < nf; ++f) {
+        double re = U.re[int64_t(f) * U.npts + p], im = U.im[int64_t(f) * U.npts + p];
+#pragma unroll
+        for (int st = 0; st < DFT_BATCH; ++st)
+            if (st < cnt) {
+                im += rph[2 * (st * nf + f) + 1] * v[st];
+                re += rph[2 * (st * nf + f)] * v[st];
             }
-        }
-    }
-}
-
-// the FP32 partial DFT sums of one unit added into its FP64 sums, then cleared
-__global__ void __launch_bounds__(256) k_dft_flush(double* re, double* im, float* pre, float* pim, int64_t n)
-{
-    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += int64_t(gridDim.x) * blockDim.x) {
-        re[t] += double(pre[t]);
-        im[t] += double(pim[t]);
-        pre[t] = 0.f;
-        pim[t] = 0.f;
+        U.re[int64_t(f) * U.npts + p] = re;
+        U.im[int64_t(f) * U.npts + p] = im;
     }
 }
 
@@ -1586,9 +1517,17 @@ struct EngineImpl {
     double *d_inv[2][3] = {}, *d_eps[2][3] = {}, *d_amp[2][3] = {}, *d_c[2][3] = {};
     PmlDev pml[2][3] = {};
     std::vector<DftDev> units[2];
-    // FP32 partial DFT sums exist (dft_partial) and hold steps not yet added to
-    // the FP64 sums (dft_dirty)
-    bool dft_partial = false, dft_dirty = false;
+    // E/H monitor sample rings: steps gathered since the last flush, and the
+    // first of them; the units they serve (half, unit)
+    int ring_cnt = 0;
+    int64_t ring_n0 = 0;
+    std::vector<std::pair<int, int>> ring_units;
+    template <typename T>
+    RingArgs<T> ring_args();
+    template <typename T>
+    void gather_dft();
+    template <typename T>
+    void flush_ring();
     void flush_dft();
     std::vector<std::vector<double>> host_freqs[2];
     double* freqs_dev[2] = {};
@@ -2025,20 +1964,20 @@ void EngineImpl::build()
                 }
                 d.npts = 0;
             }
-            // E/H samples are accumulated by the one-thread-per-point kernel after
-            // the half step: inside the z-marching kernel each monitored plane
-            // would add a dependent read-modify-write round trip per component,
-            // while the separate pass re-reads only the sampled value.  D/B
-            // samples need the polarization sums, so those stay in the half step.
+            // E/H samples are gathered after the step into the unit's ring and
+            // added into the sums in batches: inside the z-marching kernel each
+            // monitored plane would add a dependent read-modify-write round trip
+            // per component.  D/B samples need the polarization sums, so those
+            // stay in the half step.
             d.fused = d.is_db ? 1 : 0;
             if (d.npts >= (int64_t(1) << 31))
                 throw std::runtime_error("DFT monitor with 2^31 or more points");
             d.re = dalloc<double>(size_t(d.npts) * d.nf);
             d.im = dalloc<double>(size_t(d.npts) * d.nf);
-            if (fp32 && !d.fused && d.npts > 0) {
-                d.pre = dalloc<float>(size_t(d.npts) * d.nf);
-                d.pim = dalloc<float>(size_t(d.npts) * d.nf);
-                dft_partial = true;
+            if (!d.fused && d.npts > 0) {
+                d.ring = fp32 ? (void*)dalloc<float>(size_t(d.npts) * DFT_BATCH, false)
+                              : (void*)dalloc<double>(size_t(d.npts) * DFT_BATCH, false);
+                ring_units.emplace_back(h, int(units[h].size()));
             }
             units[h].push_back(d);
         }
@@ -2205,7 +2144,7 @@ void EngineImpl::build()
             }
         }
         for (auto& U : units[h])
-            if (U.fused)  // the others are accumulated by k_dft_sparse
+            if (U.fused)  // the others go through the sample ring
                 dftb += double(U.npts) * U.nf * 32.0;  // re, im: read + write, FP64
         double* t = traffic[h];
         t[0] = (2 * nreal + nsrc) * double(tsz);
@@ -2403,6 +2342,7 @@ void EngineImpl::ensure_phasors()
 {
     if (ph_start >= 0 && n < ph_start + PH_CHUNK)
         return;
+    flush_dft();  // the pending samples' phasors are rows of the current table
     ph_start = n;
     for (int h = 0; h < 2; ++h) {
         if (MF[h] == 0)
@@ -2554,55 +2494,6 @@ void EngineImpl::post_half(int h)
         B.ph = phr;
         k_sources<T><<<(SR.ntarget + 255) / 256, 256, 0, stream>>>(B, t);
         HIP_OK(hipGetLastError());
-    }
-    {
-        SparseDftArgs<T> D;
-        std::memset(&D, 0, sizeof(D));
-        for (int c = 0; c < 3; ++c)
-            D.u[c] = static_cast<const T*>(field[h * 3 + c]) + S->L.origin;
-        D.pitch = int(S->L.pitch);
-        D.plane = int(S->L.plane);
-        int64_t most = 0;
-        bool single = true;
-        for (auto& U : units[h])
-            if (!U.fused && U.npts > 0) {
-                D.units[D.nunits++] = U;
-                most = std::max(most, U.npts);
-                single = single && U.nf == 1;
-            }
-        D.ph = phr;
-        if (D.nunits > 0) {
-            constexpr int PT = 2;  // measured: 2 and 1 beat 4 (design monitors 4.2 vs 4.7 us FP32)
-            // FP32 engines keep partial sums, added into the FP64 sums every DFT_FLUSH steps
-            const int mode = !dft_partial ? 0 : (n % DFT_FLUSH == DFT_FLUSH - 1 ? 2 : 1);
-            if (single) {
-                const dim3 g(unsigned((most + 256 * PT - 1) / (256 * PT)), D.nunits);
-                if (mode == 0)
-                    k_dft_sparse1<T, PT, 0><<<g, 256, 0, stream>>>(D);
-                else if (mode == 1)
-                    k_dft_sparse1<T, PT, 1><<<g, 256, 0, stream>>>(D);
-                else
-                    k_dft_sparse1<T, PT, 2><<<g, 256, 0, stream>>>(D);
-            } else {
-                int mf = 1;
-                for (int u = 0; u < D.nunits; ++u)
-                    mf = std::max(mf, D.units[u].nf);
-                D.npb = int((most + 255) / 256);
-                D.nch = (mf + 3) / 4;
-                // a multiple of 8 workgroups, so that the XCD remap is a bijection
-                const int64_t nb = (int64_t(D.npb) * D.nch + 7) / 8 * 8;
-                if (nb > 0x7fffffff)
-                    throw std::runtime_error("DFT monitor too large: points x frequency chunks exceed one grid");
-                const dim3 g(unsigned(nb), D.nunits);
-                if (mode == 0)
-                    k_dft_sparse<T, 0><<<g, 256, 0, stream>>>(D);
-                else if (mode == 1)
-                    k_dft_sparse<T, 1><<<g, 256, 0, stream>>>(D);
-                else
-                    k_dft_sparse<T, 2><<<g, 256, 0, stream>>>(D);
-            }
-            HIP_OK(hipGetLastError());
-        }
     }
     if (!faces[h].empty()) {
         FaceArgs<T> F;
@@ -2919,24 +2810,78 @@ void EngineImpl::step_once(hipEvent_t* ev, bool timed_e, bool timed_h)
         k_probes<T><<<(nprobes + 63) / 64, 64, 0, stream>>>(P, n);
         HIP_OK(hipGetLastError());
     }
-    if (dft_partial)
-        dft_dirty = n % DFT_FLUSH != DFT_FLUSH - 1;
+    if (!ring_units.empty())
+        gather_dft<T>();
     ++n;
 }
 
+template <typename T>
+RingArgs<T> EngineImpl::ring_args()
+{
+    RingArgs<T> R;
+    std::memset(&R, 0, sizeof(R));
+    for (int c = 0; c < 6; ++c)
+        R.u[c] = static_cast<const T*>(field[c]) + S->L.origin;
+    R.pitch = int(S->L.pitch);
+    R.plane = int(S->L.plane);
+    R.nunits = int(ring_units.size());
+    R.first[0] = 0;
+    for (int q = 0; q < R.nunits; ++q) {
+        const auto& hu = ring_units[size_t(q)];
+        R.units[q] = units[hu.first][size_t(hu.second)];
+        R.half[q] = hu.first;
+        R.first[q + 1] = R.first[q] + R.units[q].npts;
+    }
+    return R;
+}
+
+// sample of this step (both halves' monitors read the final values of the step)
+template <typename T>
+void EngineImpl::gather_dft()
+{
+    if (ring_cnt == 0)
+        ring_n0 = n;
+    RingArgs<T> R = ring_args<T>();
+    R.slot = ring_cnt;
+    const int64_t total = R.first[R.nunits];
+    k_dft_gather<T><<<unsigned((total + 255) / 256), 256, 0, stream>>>(R);
+    HIP_OK(hipGetLastError());
+    if (++ring_cnt == DFT_BATCH)
+        flush_ring<T>();
+}
+
+template <typename T>
+void EngineImpl::flush_ring()
+{
+    if (ring_cnt == 0)
+        return;
+    RingArgs<T> R = ring_args<T>();
+    R.cnt = ring_cnt;
+    int64_t most = 0;
+    int nf = 1;
+    for (int h = 0; h < 2; ++h) {
+        R.ph[h] = ph[h] + size_t(ring_n0 - ph_start) * 2 * std::max(1, MF[h]);
+        R.mf[h] = std::max(1, MF[h]);
+    }
+    for (int q = 0; q < R.nunits; ++q) {
+        most = std::max(most, R.units[q].npts);
+        nf = std::max(nf, R.units[q].nf);
+    }
+    const size_t lds = size_t(ring_cnt) * nf * 2 * sizeof(double);
+    if (lds > 65536)
+        throw std::runtime_error("DFT monitor with too many frequencies for one batch");
+    k_dft_ring<T><<<dim3(unsigned((most + 255) / 256), unsigned(R.nunits)), 256, lds, stream>>>(R);
+    HIP_OK(hipGetLastError());
+    ring_cnt = 0;
+}
+
+// every gathered sample added into the FP64 sums (before any read of them)
 void EngineImpl::flush_dft()
 {
-    if (!dft_dirty)
-        return;
-    for (int h = 0; h < 2; ++h)
-        for (auto& U : units[h])
-            if (U.pre) {
-                const int64_t m = U.npts * U.nf;
-                const int64_t nb = std::min<int64_t>((m + 255) / 256, 4096);
-                k_dft_flush<<<unsigned(nb), 256, 0, stream>>>(U.re, U.im, U.pre, U.pim, m);
-                HIP_OK(hipGetLastError());
-            }
-    dft_dirty = false;
+    if (fp32)
+        flush_ring<float>();
+    else
+        flush_ring<double>();
 }
 
 void EngineImpl::step_n(int64_t nsteps)

@@ -114,9 +114,9 @@ def test_stop_conditions_match_oracle(stop):
 
 
 @pytest.mark.gpu
-def test_dft_stop_condition_fp32_partial_sums():
+def test_dft_stop_condition_fp32_storage():
     """FP32 storage: the DFT-decay stop condition reads the monitor after adding
-    the FP32 partial sums into the FP64 sums each step of its windows.  Same stop
+    the gathered samples into the FP64 sums each step of its windows.  Same stop
     step as the FP64 oracle (the last window's variation, 9.97e-4 against 1e-3,
     is far outside FP32 rounding) and the final monitor within 1e-5."""
     stop = {"type": "dft", "center": [0, 0, 3], "size": [16, 16, 0], "field component": "Ez",

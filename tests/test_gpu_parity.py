@@ -165,31 +165,34 @@ def test_many_frequency_monitors_match_oracle():
 
 
 @pytest.mark.parametrize("nf", [9, 1])
-def test_fp32_partial_dft_sums_are_flushed(nf):
-    """FP32 engines sum E/H monitors in FP32 over 16 steps before adding into the
-    FP64 sums.  120 steps leave 8 steps in the partial sums when they are read:
-    FP32 must still match FP64 closely (a lost partial would miss 8 steps of a
-    pulse that is still passing), for the general and single-frequency kernels,
-    read once mid-run and once at the end."""
+def test_dft_sample_batches_are_flushed(nf):
+    """E/H monitors gather 16 steps of samples before adding them into the FP64
+    sums.  Read after 75 steps (11 samples pending) and after 120 (8 pending),
+    both storage types must equal the oracle: a lost or doubled batch would miss
+    or repeat steps of a pulse that is still passing."""
     freqs = list(np.linspace(0.05, 0.15, nf)) if nf > 1 else [0.1]
     cfg = W.vacuum_dipole(n=32, steps=120, pml_cells=6, freqs=freqs, plane_z=5)
     cfg["fields output"] += [W.dft((3, 0, 0), (0, 32, 32), freqs, "Ez", "volume fields dft 1")]
-    got = {}
-    for prec in ("fp64", "fp32"):
+    ora = OracleSim(cfg)
+    ora.run({"type": "time", "time": 37.5})
+    ref = [[ora.dft_output(k) for k in range(2)]]
+    ora.run(cfg["stop condition"])
+    ref.append([ora.dft_output(k) for k in range(2)])
+    for prec, tol in (("fp64", TOL64), ("fp32", 1e-5)):
         cfg["precision"] = prec
         sim = LibSim(cfg)
-        sim.run({"type": "time", "time": 37.5})  # 75 steps: 11 left in the partial sums
-        mid = [sim.dft(k) for k in range(2)]
+        sim.run({"type": "time", "time": 37.5})
+        got = [[sim.dft(k) for k in range(2)]]
         sim.run(cfg["stop condition"])
-        got[prec] = (mid, [sim.dft(k) for k in range(2)])
+        got.append([sim.dft(k) for k in range(2)])
         sim.close()
-    for stage in range(2):
-        for k in range(2):
-            a, b = got["fp64"][stage][k], got["fp32"][stage][k]
-            assert np.abs(a["real"]).max() > 0
-            for part in ("real", "imag"):
-                e = rel_l2(b[part], a[part])
-                assert e <= 1e-5, (nf, stage, k, part, e)
+        for stage in range(2):
+            for k in range(2):
+                a, b = got[stage][k], ref[stage][k]
+                assert np.abs(b["real"]).max() > 0
+                for part in ("real", "imag"):
+                    e = rel_l2(a[part], b[part])
+                    assert e <= tol, (prec, nf, stage, k, part, e)
 
 
 def test_continuation_equals_single_run():
@@ -243,31 +246,54 @@ def test_full_size_dispersive_fp32_against_fp64():
         assert e <= 1e-4
 
 
-def test_full_size_voxel_mix_512_fp32_against_fp64():
-    """The 512^3 voxel mix at its real shape (three-pole Lorentz and Drude voxels,
-    compact polarization, CPML interior/shell launches), with two Ez DFT planes
-    added, 300 steps: FP32 storage against FP64 storage on the monitors and on
-    point probes (the FP64 engine equals the oracle at 1e-10 on the small cases)."""
-    cfg = W.voxel_mix(n=512, precision="fp64", nfreq=4, nplanes=2)
-    cfg["stop condition"] = {"type": "time", "time": 150.0}
-    pts = [(0.5, 0.0, -140.0), (30.0, -20.0, -100.0), (-60.0, 45.0, -180.0)]
+def test_voxel_mix_64_frequency_planes_match_oracle():
+    """The large-run recipe (three-pole Lorentz and Drude voxels, compact
+    polarization, an Ez sheet normal to x) at a size the oracle runs, with its
+    monitor set: 8 z-normal Ez planes at half-integer z x 64 frequencies in
+    [0.05, 0.15].  FP64 against the oracle at the parity bar; the planes carry
+    the pulse (their largest |DFT| is printed)."""
+    cfg = W.voxel_mix(n=32, voxel=4, pml_cells=5, nfreq=64, nplanes=8, precision="fp64", source_offset=7)
+    cfg["stop condition"] = {"type": "time", "time": 60.0}
+    worst = _compare(cfg, label="voxel mix 32^3, 8 planes x 64 freqs")
+    ora = OracleSim(cfg, strict_pool=False)
+    ora.run_steps(120)
+    peak = max(np.abs(ora.dft_output(k)["real"]).max() for k in range(8))
+    print("largest |DFT| %.3e, worst rel L2 %.3e" % (peak, worst))
+    assert peak > 0.1
+
+
+def _fp32_vs_fp64(cfg, steps, nmon):
     out = {}
     for prec in ("fp64", "fp32"):
         cfg["precision"] = prec
         sim = LibSim(cfg)
-        assert sim.run(cfg["stop condition"]) == 300
-        out[prec] = ([sim.dft(k) for k in range(2)], [sim.probe(p, "Ez") for p in pts])
+        sim.step(steps)
+        out[prec] = [sim.dft(k) for k in range(nmon)]
         sim.close()
-    (d64, p64), (d32, p32) = out["fp64"], out["fp32"]
-    reached = False
-    for k in range(2):
-        scale = np.abs(d64[k]["real"]).max()
-        reached = reached or scale > 0
-        if scale > 0:
-            e = max(rel_l2(d32[k][part], d64[k][part]) for part in ("real", "imag"))
-            print("monitor %d: fp32 vs fp64 rel L2 %.3e, |DFT| max %.3e" % (k, e, scale))
-            assert e <= 1e-4
-    assert reached
-    p64, p32 = np.array(p64), np.array(p32)
-    assert np.abs(p64).max() > 0
-    assert np.abs(p32 - p64).max() <= 1e-4 * np.abs(p64).max()
+    errs, peaks = [], []
+    for k in range(nmon):
+        a, b = out["fp64"][k], out["fp32"][k]
+        peaks.append(float(np.abs(a["real"] + 1j * a["imag"]).max()))
+        errs.append(max(rel_l2(b[part], a[part]) for part in ("real", "imag")))
+        print("monitor %d: |DFT| max %.3e, fp32 vs fp64 rel L2 %.3e" % (k, peaks[-1], errs[-1]))
+    return errs, peaks
+
+
+def test_full_size_voxel_mix_512_fp32_against_fp64():
+    """The 512^3 voxel mix at its real shape with two Ez DFT planes x 4
+    frequencies, 300 steps: FP32 storage against FP64 storage on monitors that
+    the pulse has reached (|DFT| of order one)."""
+    cfg = W.voxel_mix(n=512, precision="fp64", nfreq=4, nplanes=2)
+    errs, peaks = _fp32_vs_fp64(cfg, 300, 2)
+    assert min(peaks) > 0.5
+    assert max(errs) <= 1e-4
+
+
+def test_full_size_voxel_mix_640_eight_planes_64_freqs_fp32_against_fp64():
+    """The large single-GPU run at its real shape: 640^3 voxel mix, 16-cell CPML,
+    8 Ez planes x 64 frequencies, 300 steps: FP32 storage (FP64 sums) against
+    FP64 storage on every monitor, each carrying the pulse."""
+    cfg = W.voxel_mix(n=640, precision="fp64", nfreq=64, nplanes=8)
+    errs, peaks = _fp32_vs_fp64(cfg, 300, 8)
+    assert min(peaks) > 0.5
+    assert max(errs) <= 1e-4
