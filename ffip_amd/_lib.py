@@ -57,6 +57,8 @@ _SIGS = {
     "ffip_transpose": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                         ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
     "ffip_sim_destroy": ([ctypes.c_void_p], None),
+    "ffip_to_convolve_noalloc": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                  ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "ffip_stream_bandwidth": ([ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p],
                               ctypes.c_int),
     "ffip_to_convolve": ([ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,

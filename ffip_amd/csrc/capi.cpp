@@ -17,6 +17,8 @@
 
 namespace ffip_amd {
 void stream_bandwidth(int R, int W, size_t bytes, int reps, double out[2]);
+void to_convolve_noalloc(const double* in, int64_t n0, int64_t n1, const double* filt, int64_t f0, int64_t f1,
+                         bool transpose, double* out, void* stream);
 void to_convolve(const double* in, int64_t n0, int64_t n1, const double* filt, int64_t f0, int64_t f1, bool transpose,
                  double* out);
 }
@@ -676,6 +678,16 @@ int ffip_to_convolve(const double* in, int64_t n0, int64_t n1, const double* fil
     if (!in || !filt || !out)
         return fail(FFIP_EINVAL, "null argument");
     return guard([&] { to_convolve(in, n0, n1, filt, f0, f1, transpose != 0, out); });
+}
+
+int ffip_to_convolve_noalloc(const double* in, int64_t n0, int64_t n1, const double* filt, int64_t f0, int64_t f1,
+                             int transpose, double* out, void* stream)
+{
+    if (!in || !filt || !out)
+        return fail(FFIP_EINVAL, "null argument");
+    return guard([&] {
+        to_convolve_noalloc(in, n0, n1, filt, f0, f1, transpose != 0, out, stream);
+    });
 }
 
 int ffip_stream_bandwidth(int reads, int writes, size_t bytes_per_array, int reps, double out[2])

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -116,6 +117,61 @@ void to_convolve(const double* in, int64_t n0, int64_t n1, const double* filt, i
         throw;
     }
     cleanup();
+}
+
+// ---------------------------------------------------------------- allocation-free form
+// Window weights of the allocation-free entry point live in a file-scope device
+// array (one per GPU, held until the process ends).  Host code must not pass
+// the array's name to a kernel: it is a host-side stand-in.  Its device address
+// is looked up once per device and kept.
+constexpr int64_t NOALLOC_MAX = int64_t(1) << 21;  // elements (16 MiB of FP64 weights)
+static __device__ __align__(256) double scratch[NOALLOC_MAX];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64] = {};
+
+static double* scratch_ptr()
+{
+    int dev = 0;
+    FL_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64)
+        throw std::runtime_error("device index out of range for the filter scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        FL_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// TO_convolve / TO_convolve_transpose on device arrays, asynchronously on
+// `stream`, with no allocation: the window weights use the device's scratch
+// array (calls that overlap on one device race on it).
+void to_convolve_noalloc(const double* in, int64_t n0, int64_t n1, const double* filt, int64_t f0, int64_t f1,
+                         bool transpose, double* out, void* stream_)
+{
+    const hipStream_t stream = static_cast<hipStream_t>(stream_);
+    if (n0 < 1 || n1 < 1 || n0 * n1 >= (int64_t(1) << 31))
+        throw std::runtime_error("array size out of range");
+    if (n0 * n1 > NOALLOC_MAX)
+        throw std::runtime_error("array of " + std::to_string(n0 * n1) + " elements exceeds the allocation-free limit of " +
+                                 std::to_string(NOALLOC_MAX) + " elements");
+    if (f0 % 2 == 0 || f1 % 2 == 0)
+        throw std::runtime_error("filter shape is not odd");
+    if (f0 != f1)
+        throw std::runtime_error("filter must be square (the reference indexes both axes with the first radius)");
+    const int r = int(f0 / 2);
+    double* w = scratch_ptr();
+    dim3 block(32, 8);
+    dim3 grid(unsigned((n1 + 31) / 32), unsigned((n0 + 7) / 8));
+    k_window_weight<<<grid, block, 0, stream>>>(int(n0), int(n1), filt, r, w);
+    FL_OK(hipGetLastError());
+    if (transpose)
+        k_to_convolve_t<<<grid, block, 0, stream>>>(int(n0), int(n1), in, filt, r, w, out);
+    else
+        k_to_convolve<<<grid, block, 0, stream>>>(int(n0), int(n1), in, filt, r, w, out);
+    FL_OK(hipGetLastError());
 }
 
 }  // namespace ffip_amd

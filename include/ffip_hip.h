@@ -157,6 +157,18 @@ int ffip_transpose(const double *gx, int64_t nx, const double *gy, int64_t ny, c
 int ffip_to_convolve(const double *in, int64_t n0, int64_t n1, const double *filt, int64_t f0, int64_t f1,
                      int transpose, double *out);
 
+/* Allocation-free form of ffip_to_convolve.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  in, filt and out are DEVICE
+ * arrays ([n0][n1], [f0][f1] with odd, equal f0 and f1); the call is
+ * asynchronous on `stream` (a hipStream_t, NULL = default stream) and allocates
+ * nothing: the window weights live in a fixed device array of 2^21 elements
+ * (16 MiB per GPU, held until the process ends), so n0 * n1 may not exceed
+ * 2^21 (FFIP_EINVAL otherwise).  Calls that overlap on one GPU share that array
+ * and race; results equal ffip_to_convolve's bit for bit. */
+int ffip_to_convolve_noalloc(const double *in, int64_t n0, int64_t n1, const double *filt, int64_t f0, int64_t f1,
+                             int transpose, double *out, void *stream);
+
 /* ------------------------------------------------------------------------
  * Design surrogate kernels.  No counterpart in the reference: a 3-D
  * convolutional network over the design density, trained and evaluated on the
