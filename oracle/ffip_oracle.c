@@ -25,8 +25,9 @@
  *
  * Pinning (what fixes each function independently of this restatement) is listed
  * in tests/test_oracle_pins.py: discrete plane-wave dispersion in a periodic box,
- * the discrete susceptibility closed form, the CPML geometric series, the DFT of a
- * known series, the Gaussian1 closed form and the reference's Yee3 known answers.
+ * the discrete susceptibility closed form, the CPML reflection, the DFT of a
+ * known series, the Gaussian1 closed form, the reference's Yee3 known answers,
+ * mirror planes against full grids, and the current-sheet Green's function.
  *
  * Threads (OpenMP, osim_set_threads): the per-point loops of each phase write
  * distinct points, as one MPI rank's loops do, so any thread count gives the

@@ -10,6 +10,14 @@ mathematics that does not go through the restated formula:
   * a3 CPML                   -> measured normal-incidence reflection
   * a5 ADE coefficients       -> discrete susceptibility closed form D/E = eps(z)
   * a6 density interpolation  -> textbook trilinear interpolation (scipy)
+  * PEC/PMC ghost faces        -> a half grid with a mirror plane equals the full
+                                  grid driven by mirror-symmetric sources
+  * volume-source amplitude    -> discrete 1-D Green's function of a current sheet
+                                  and the area scaling of a stretched sheet
+  * stop-condition windows     -> replay of the window logic from a recorded series
+Each of the last three was checked against copies of the oracle with a slip
+planted (multiplier sign, dropped or doubled spacing factor, norm read before
+the step, running maximum not folded): every slip fails its pin.
 """
 import math
 
@@ -298,3 +306,299 @@ def test_threads_give_the_same_bits():
     for a, b in zip(out[0][0] + out[0][1], out[1][0] + out[1][1]):
         assert np.array_equal(a, b)
     assert any(np.abs(f).max() > 0 for f in out[0][1])
+
+
+# --------------------------------------------------------------- mirror planes
+def _lattice(sim, comp):
+    """Doubled-grid coordinates (x, y, z arrays) of `comp` inside the grid, the
+    axes of ``sim.field(comp)`` reversed."""
+    p1, p2 = O.component_interior(sim.grid.grid_p1, sim.grid.grid_p2, O.CTYPE[comp])
+    return [np.arange(p1[a], p2[a] + 1, 2) for a in range(3)]
+
+
+def _dipoles(cfg, dips):
+    cfg["sources"] = [{"type": "volume source", "center": list(map(float, c)), "size": [0.0, 0.0, 0.0],
+                       "amplitude": float(a), "field component": comp, "function": W.gaussian1(0.15)}
+                      for c, comp, a in dips]
+    return cfg
+
+
+class _FlipSide(O.OracleSim):
+    """The oracle with one mirror face's multiplier negated: the slip a pin
+    must catch."""
+    flip = (0, 0)
+
+    def _build_sync(self):
+        ops = super()._build_sync()
+        a, side = self.flip
+        src, dst, mult = ops[a][side]
+        ops[a][side] = (src, dst, -mult)
+        return ops
+
+
+# (axis, size, periodic, full-grid dipoles, half-grid dipoles, phase)
+_MIRRORS = {
+    # Ez on the plane x = 0: Ez, Ey, Hx even in x -> tangential H vanishes (PMC)
+    "x-even": (0, (12, 10, 10), None, [((0, 0.3, 0.25), "Ez", 1)], [((0, 0.3, 0.25), "Ez", 1)], 1),
+    # Ez pair odd in x -> tangential E vanishes (PEC)
+    "x-odd": (0, (12, 10, 10), None, [((2.25, 0.3, 0.25), "Ez", 1), ((-2.25, 0.3, 0.25), "Ez", -1)],
+              [((2.25, 0.3, 0.25), "Ez", 1)], -1),
+    # Ez pointing through the plane z = 0 and even in z: the mirror image of a
+    # polar current flips its normal part, so this is the PEC case
+    "z-normal": (2, (10, 10, 12), None, [((0.3, 0.25, 0), "Ez", 1)], [((0.3, 0.25, 0), "Ez", 1)], -1),
+    # Ex on y = 0, even in y: tangential, PMC
+    "y-even": (1, (10, 12, 10), None, [((0.25, 0, 0.3), "Ex", 1)], [((0.25, 0, 0.3), "Ex", 1)], 1),
+}
+
+
+def _mirror_case(axis, size, periodic, full_dips, half_dips, phase, steps=40, sim_cls=None):
+    full = _dipoles(W.base(size, 0), full_dips)
+    half = _dipoles(W.base(size, 0), half_dips)
+    if periodic:
+        full["periodic"] = list(periodic)
+        half["periodic"] = list(periodic)
+    half["symmetry"] = [{"direction": "xyz"[axis], "phase factor": phase}]
+    a, b = O.OracleSim(full), (sim_cls or O.OracleSim)(half)
+    a.run_steps(steps)
+    b.run_steps(steps)
+    worst, scale = 0.0, 0.0
+    for comp in O.EH_ORDER:
+        fa, fb = a.field(comp), b.field(comp)
+        la, lb = _lattice(a, comp), _lattice(b, comp)
+        # the half grid's lattice is the full grid's at coordinates >= 0 on `axis`
+        keep = la[axis] >= 0
+        assert np.array_equal(la[axis][keep], lb[axis])
+        sel = [slice(None)] * 3
+        sel[2 - axis] = keep
+        fa = fa[tuple(sel)]
+        assert fa.shape == fb.shape
+        worst = max(worst, float(np.abs(fa - fb).max()))
+        scale = max(scale, float(np.abs(fa).max()))
+    return worst, scale
+
+
+@pytest.mark.parametrize("name", list(_MIRRORS))
+def test_mirror_plane_equals_full_grid(name):
+    """PEC/PMC ghost faces (src/fields.cpp:464-482, multiplier phase or -phase by
+    the parity of the face coordinate): a half grid with a `symmetry` plane at 0
+    reproduces, on its half, the full grid driven by the mirror-symmetric
+    sources, and the other phase does not."""
+    axis, size, per, fd, hd, phase = _MIRRORS[name]
+    worst, scale = _mirror_case(axis, size, per, fd, hd, phase)
+    assert scale > 1e-3
+    assert worst <= 1e-13 * scale, (worst, scale)
+    wrong, _ = _mirror_case(axis, size, per, fd, hd, -phase)
+    assert wrong > 1e-2 * scale
+
+
+def test_mirror_plane_at_odd_coordinate():
+    """The positive side of a periodic mirrored axis mirrors about its last
+    plane, here at an odd doubled coordinate (13 cells), where the even-lattice
+    components (Ey, Ez, Hx) are the ones copied: their multiplier is +phase.  The
+    full grid is a periodic 13-cell box holding the mirror image; a slip that
+    negates either face's multiplier fails."""
+    fd = [((0, 0.3, 0.25), "Ez", 1)]
+    args = (0, (13, 10, 10), (1, 0, 0), fd, fd, 1)
+    worst, scale = _mirror_case(*args, steps=60)
+    assert scale > 1e-3
+    assert worst <= 1e-13 * scale, (worst, scale)
+    for side in (0, 1):
+        cls = type("Flip%d" % side, (_FlipSide,), {"flip": (0, side)})
+        wrong, _ = _mirror_case(*args, steps=60, sim_cls=cls)
+        assert wrong > 1e-2 * scale, side
+
+
+# ----------------------------------------------------------- sheet amplitudes
+def _sheet_column(P, size, nz=120, pml_cells=16, steps=1600, zmon=20, freqs=(0.03, 0.05, 0.07)):
+    """An Ex sheet of the given (x, y) size in a column periodic in x, y with
+    period P cells and CPML on z; returns the cross-section mean of the Ex DFT
+    on a plane zmon cells past the sheet over one period of distinct points
+    (the (0, 0) Floquet mode), and the oracle."""
+    cfg = W.periodic_plane_wave(nz=nz, nxy=P, steps=steps, freqs=freqs, planes=(0,), pml_cells=pml_cells)
+    zs = -nz / 2 + pml_cells + 4
+    src = cfg["sources"][0]
+    src["center"] = [0.0, 0.0, zs]
+    src["size"] = [float(size[0]), float(size[1]), 0.0]
+    src["function"] = W.gaussian1(0.05, cutoff=8)
+    cfg["fields output"] = [W.dft((0, 0, zs + zmon), (P, P, 0), freqs, "Ex", "m")]
+    sim = O.OracleSim(cfg)
+    sim.run(cfg["stop condition"])
+    d = sim.dft_output(0)
+    # x: the P odd-lattice points inside (the closure adds one outside at each
+    # end); y: P + 1 even-lattice points, the last a copy of the first across
+    # the periodic boundary
+    xs = (d["x"] > -P / 2) & (d["x"] < P / 2)
+    ys = (d["y"] >= -P / 2) & (d["y"] < P / 2)
+    re = d["real"][:, 0][:, ys][:, :, xs]
+    im = d["imag"][:, 0][:, ys][:, :, xs]
+    assert re.shape[1:] == (P, P)
+    return re.mean(axis=(1, 2)) + 1j * im.mean(axis=(1, 2)), sim
+
+
+def test_sheet_amplitude_is_a_current_density():
+    """Volume-source amplitudes are current densities (src/simulation.cpp:252-262).
+
+    1. Absolute: a 6 x 6-cell sheet (7 x 7 dipoles one cell apart, each split
+       half-and-half between two Ex points) in a 10 x 10 periodic column holds
+       a mean current of 0.49 per lattice point.  Its (0, 0) mode must equal the
+       discrete 1-D Green's function of a uniform sheet of that density,
+           E(w) = -J(w) dx^2 sin(w dt/2) / (dt sin(k dx)) exp(-i k d),
+       with k from the Yee dispersion relation and J(w) the DFT of the Gaussian1
+       samples at t = (n + 1/2) dt.  (Derived from the two 1-D Yee updates with
+       a current in the E update; its continuum limit is E = -eta J_s / 2.)
+       Scaling by the dipole spacing in doubled-grid units (2, not 1) would
+       make this 4x too large.
+    2. Relative: stretching the sheet to 6.4 x 6.4 cells keeps 7 x 7 dipoles but
+       spreads them 6.4/6 cells apart; at the same density the radiated mode
+       grows with the area, (6.4/6)^2, not with the dipole count (ratio 1)."""
+    freqs = (0.03, 0.05, 0.07)
+    b, sim = _sheet_column(10, (6, 6), freqs=freqs)
+    a, _ = _sheet_column(10, (6.4, 6.4), freqs=freqs)
+    dt = sim.dt
+    width = 1 / (2 * PI * 0.05)
+    t = (np.arange(1600) + 0.5) * dt
+    on = (t > 0) & (t < 2 * width * 8)
+    arg = (t - 8 * width) / width
+    g = np.where(on, arg * np.exp(-arg * arg / 2), 0.0)
+    for fi, f in enumerate(freqs):
+        w = 2 * PI * f
+        J = np.sum(g * np.exp(-1j * w * t))
+        k = _discrete_k(f)
+        expect = -0.49 * J * math.sin(w * dt / 2) / (dt * math.sin(k)) * np.exp(-1j * k * 20)
+        assert abs(b[fi] - expect) <= 2e-3 * abs(expect), (f, b[fi], expect)
+        ratio = a[fi] / b[fi]
+        assert abs(ratio - (6.4 / 6) ** 2) <= 1e-9, (f, ratio)
+
+
+# ------------------------------------------------------------ stop conditions
+def _windows(n_turnoff_time, dt, interval, n_max):
+    """Step counts closing each stop-condition window: window 0 ends at the
+    first step count n with n dt >= T, window k at the first with
+    n dt >= T + k * interval (the loop tests the count before stepping)."""
+    ends = []
+    k = 0
+    n = 0
+    while n <= n_max:
+        lim = n_turnoff_time + k * interval
+        while not n * dt >= lim:
+            n += 1
+        ends.append(n)
+        k += 1
+        if n >= n_max:
+            break
+    return ends
+
+
+def _differs(slipped, got):
+    """A replay with a slipped window differs from the true one in its window
+    count or in some window's (max, min) by more than rounding."""
+    if len(slipped) != len(got):
+        return True
+    a, b = np.array(slipped)[:, :2], np.array(got)[:, :2]
+    return bool(np.abs(a - b).max() > 1e-6 * np.abs(b).max())
+
+
+def _stop_cfg(interval, var_or_decay, kind, n=16, at=(1.0, 1.0, 0.5)):
+    """Ez dipole at the centre of an n^3 box with CPML; the stop condition
+    watches the Ez lattice point `at`."""
+    cfg = W.base((n, n, n), 4 if n < 24 else 6)
+    cfg["sources"] = [{"type": "volume source", "center": [0.0, 0.0, 0.0], "size": [0.0, 0.0, 0.0],
+                       "amplitude": 1.0, "field component": "Ez", "function": W.gaussian1(0.1)}]
+    cfg["fields output"] = [W.dft(at, (0, 0, 0), [0.1], "Ez", "m")]
+    if kind == "dft":
+        cfg["stop condition"] = {"type": "dft", "center": list(at), "size": [0.0, 0.0, 0.0],
+                                 "field component": "Ez", "frequency": 0.1,
+                                 "time interval examined": interval, "variation": var_or_decay}
+    else:
+        cfg["stop condition"] = {"type": "decay", "position": list(at), "field component": "Ez",
+                                 "time interval examined": interval, "decayed by": var_or_decay}
+    return cfg
+
+
+def _probe_trace(cfg, nsteps, at=(1.0, 1.0, 0.5)):
+    """Ez after every step at the monitored lattice point, from the C loop's probe
+    recorder (the DFT and probe paths are pinned above)."""
+    sim = O.OracleSim(cfg, probes=[(at, "Ez")], trace_steps=nsteps)
+    sim.run_steps(nsteps)
+    return sim.trace[:, 0].copy(), sim
+
+
+def test_dft_stop_condition_windows():
+    """run_until_dft (src/simulation.cpp:782-842) replayed from a recorded
+    series: after the source turns off, windows of `interval` close at the first
+    step count reaching T + k*interval; each window's spread (max - min) / max of
+    the monitor's DFT norm |sum_{m<=n} x_m exp(-i w m dt)|^2, taken after every
+    step of the window, decides whether to go on.  Windows shifted by one sample,
+    or norms read before the step, give other numbers."""
+    interval, var = 10.0, 5e-3
+    cfg = _stop_cfg(interval, var, "dft")
+    ref = O.OracleSim(cfg)
+    taken = ref.run(cfg["stop condition"])
+    log = ref.stop_log
+    x, sim = _probe_trace(cfg, taken + 200)
+    dt = sim.dt
+    n = np.arange(1, x.size + 1)
+    S = np.cumsum(x * np.exp(-2j * PI * 0.1 * n * dt))
+    norm = np.abs(S) ** 2  # norm[i]: after step i + 1
+    T = 2 * (1 / (2 * PI * 0.1)) * 4.0  # Gaussian1 end time: 2 width cutoff
+    assert ref.turnoff_time == pytest.approx(T, rel=1e-15)
+    ends = _windows(T, dt, interval, x.size - 1)
+
+    def replay(shift):
+        out = []
+        for k in range(1, len(ends)):
+            w = norm[ends[k - 1] + shift:ends[k] + shift]
+            mx, mn = w.max(), w.min()
+            out.append((mx, mn, (mx - mn) / mx))
+            if not out[-1][2] > var:
+                return out, ends[k]
+        raise AssertionError("series too short")
+
+    got, stop = replay(0)
+    assert len(got) >= 3  # several windows exercised
+    assert taken == stop
+    # the spread is a difference of nearly equal norms: compare it absolutely
+    np.testing.assert_allclose(np.array(log)[:, :2], np.array(got)[:, :2], rtol=1e-10)
+    np.testing.assert_allclose(np.array(log)[:, 2], np.array(got)[:, 2], rtol=0, atol=1e-12)
+    for shift in (-1, 1):
+        assert _differs(replay(shift)[0], got)
+
+
+def test_decay_stop_condition_windows():
+    """run_until_fields_decayed (src/simulation.cpp:844-901) replayed from a
+    recorded series: the running maximum of x^2 starts over the source-on steps;
+    each window's maximum is folded into it, and the run stops at the first
+    window whose maximum is at most `decayed by` times the running maximum.  The
+    point watched is 10 cells from the source, so the pulse peaks there after
+    the source has turned off and the folding decides when the run stops."""
+    interval, decay_by = 4.0, 1e-3
+    at = (10.0, 1.0, 0.5)
+    cfg = _stop_cfg(interval, decay_by, "decay", n=32, at=at)
+    ref = O.OracleSim(cfg)
+    taken = ref.run(cfg["stop condition"])
+    log = ref.stop_log
+    x, sim = _probe_trace(cfg, taken + 200, at)
+    dt = sim.dt
+    sq = x * x
+    T = 2 * (1 / (2 * PI * 0.1)) * 4.0
+    ends = _windows(T, dt, interval, x.size - 1)
+
+    def replay(shift):
+        top = sq[:ends[0] + shift].max()
+        out = []
+        for k in range(1, len(ends)):
+            cur = sq[ends[k - 1] + shift:ends[k] + shift].max()
+            top = max(top, cur)
+            out.append((cur, top, cur / top))
+            if not out[-1][2] > decay_by:
+                return out, ends[k]
+        raise AssertionError("series too short")
+
+    got, stop = replay(0)
+    assert len(got) >= 3
+    assert taken == stop
+    # the first window's peak exceeds the source-on peak: folding matters
+    assert sq[:ends[0]].max() < got[0][0]
+    np.testing.assert_allclose(np.array(log), np.array(got), rtol=1e-10)
+    for shift in (-1, 1):
+        assert _differs(replay(shift)[0], got)
