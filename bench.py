@@ -62,6 +62,17 @@ CPU_SAMPLE = {"t512": lambda: __import__("ffip_amd.workloads", fromlist=["voxel_
               "c5": lambda: __import__("ffip_amd.workloads", fromlist=["voxel_mix"]).voxel_mix(n=128, nfreq=64)}
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_threads():
     """Host threads for the CPU baseline: the cores this process may run on."""
     try:
@@ -70,21 +81,35 @@ def cpu_threads():
         return max(1, os.cpu_count() or 1)
 
 
+# the reference's own 1-core rate on the 64^3 vacuum dipole (8-cell CPML, one
+# Ez plane x 3 freqs), g++ -O3 -march=native on an Intel Xeon @ 2.7 GHz
+REFERENCE_VACUUM64_1CORE = 12.08
+
+
 def cpu_baseline(cfg, ds, steps, cells, threads):
-    """Time the oracle's C hot loop (FP64) for `steps` steps after its set-up, on
-    the same configuration, first on one thread and then on `threads`
-    (Mcell-updates/s for each)."""
-    from oracle.oracle import OracleSim, set_threads
+    """Time the oracle's C hot loop (FP64, the -O3 timing build) for `steps`
+    steps after its set-up, on the same configuration, first on one thread and
+    then on `threads` (Mcell-updates/s for each)."""
+    from oracle.oracle import OracleSim, set_threads, use_timing_build
+    use_timing_build()
     cfg = dict(cfg)
     cfg["precision"] = "fp64"
     sim = OracleSim(cfg, ds, strict_pool=False)
     rates = []
     for n in (1, threads):
         set_threads(n)
+        sim.run_steps(1)  # first touch of the thread team's pages
         t0 = time.perf_counter()
         sim.run_steps(steps)
         rates.append(cells * steps / (time.perf_counter() - t0) / 1e6)
     return rates
+
+
+def cpu_calibration(steps=100):
+    """1-thread rate of the timing build on the 64^3 vacuum dipole, the case the
+    reference's own 1-core rate is quoted on."""
+    from ffip_amd import workloads as W
+    return cpu_baseline(W.vacuum_dipole(64), None, steps, 64 ** 3, 1)[0]
 
 
 def design_gradient(steps):
@@ -213,11 +238,17 @@ def main():
         else:
             v1, vn = cpu_baseline(cfg, ds, args.cpu_steps, cells, nth)
             what = "same workload and grid"
+        c1 = cpu_calibration()
         cpu = {"value": round(vn, 3), "unit": "Mcell-updates/s", "cores": nth, "kind": "port",
                "value_1_thread": round(v1, 3),
-               "sample": "%s, %d steps after set-up on 1 thread and then %d on %d threads, FP64 oracle "
-                         "(C restatement of the reference hot loop, OpenMP over each phase's points) on %s"
-                         % (what, args.cpu_steps, args.cpu_steps, nth, platform.processor() or platform.machine())}
+               "calibration": {"vacuum64_1_thread": round(c1, 3),
+                               "reference_vacuum64_1_core": REFERENCE_VACUUM64_1CORE,
+                               "ratio": round(c1 / REFERENCE_VACUUM64_1CORE, 3),
+                               "note": "the reference's rate was measured on another host (Xeon @ 2.7 GHz)"},
+               "sample": "%s, %d steps after set-up on 1 thread and then %d on %d threads (all cores this "
+                         "process may use), FP64 oracle (C restatement of the reference hot loop, gcc -O3 "
+                         "-march=x86-64-v3, OpenMP over each phase's points) on %s"
+                         % (what, args.cpu_steps, args.cpu_steps, nth, cpu_model())}
 
     line = {"metric": "Mcell-updates/s", "value": round(value, 1), "unit": "Mcell-updates/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),

@@ -427,12 +427,26 @@ class GeneralMediumBox(TwoMediumBox):
 _LIB = None
 
 
+_LIB_NAME = "liboracle.so"
+
+
+def use_timing_build():
+    """Run the C loop from oracle/liboracle_timing.so (-O3, x86-64-v3, contraction
+    on) instead of the parity build.  For the CPU-baseline timing only: call it
+    before the first simulation of the process; results then differ from the
+    parity build's by rounding."""
+    global _LIB_NAME
+    if _LIB is not None and _LIB_NAME != "liboracle_timing.so":
+        raise RuntimeError("the parity build of the oracle is already loaded in this process")
+    _LIB_NAME = "liboracle_timing.so"
+
+
 def _lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboracle.so")
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), _LIB_NAME)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built; run `make -C oracle`")
+            raise RuntimeError("oracle/%s is not built; run `make -C oracle`" % _LIB_NAME)
         _LIB = ctypes.CDLL(path)
         _LIB.osim_run_steps.argtypes = [ctypes.c_void_p, ctypes.c_int64]
         _LIB.osim_run_until_time.argtypes = [ctypes.c_void_p, ctypes.c_double]
