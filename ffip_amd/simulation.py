@@ -1,52 +1,176 @@
-"""Simulation front end (the public names of ffip/simulation.py) on the MI355X
-engine.
+"""Simulation front end on the MI355X engine.
 
-The reference's ``Simulation.run`` writes config.json and an input HDF5 file,
-runs ``mpirun -np N run_sim_json`` and reads the DFT groups back from the output
-HDF5 file (ffip/simulation.py:404-431).  Here the same JSON document and the same
-datasets go in-process to libffip_hip.so through its C ABI, and results are read
-back into the same objects.  ``np`` and ``decomposition`` are accepted and
-ignored: one GPU holds the whole grid.
+Scripts written for the reference's Python package use the names of
+ffip/simulation.py: Simulation and its output objects (DFT monitors, flux planes
+and boxes, geometry-info requests), the stop conditions, PML and Symmetry, and
+the adjoint helpers (Adjoint_Source, Adjoint_Flux, Adjoint_Volume,
+field_match_objective).  Those names, their arguments and defaults, the
+attributes scripts read, the JSON document (src/simulation.cpp:58-188) and the
+output group names are kept.  How a run happens is this package's own: the
+reference writes config.json and an HDF5 input file, starts
+``mpirun -np N run_sim_json`` and reads an HDF5 output file back
+(ffip/simulation.py:404-431); here the same document and arrays go in-process
+to libffip_hip.so through its C ABI, one GPU holds the whole grid, and the
+results are read back from device memory into the same objects.  ``np`` and
+``decomposition`` are accepted and have no effect.
+
+Output objects read their results from a mapping of group name to arrays
+("real", "imag", "x", "y", "z", "f" for a DFT group; "x", "y", "z" for a
+geometry-info group), which is what an HDF5 output file of run_sim_json holds
+too, so ``read`` accepts either.
 """
 from __future__ import annotations
 
-import json
-from copy import copy, deepcopy
+from copy import deepcopy
 from math import pi
 
 import numpy as np
 from scipy.interpolate import RegularGridInterpolator
 
 from ._lib import LibSim, transpose
-from .geom import Medium, Two_Medium_Box, Vector3, cmp_shape
+from .geom import Medium, Two_Medium_Box, Vector3, cmp_shape, metadata_to_pts
 from .source import Inhom_Source
+
+__all__ = ["PML", "Adjoint_Flux", "Adjoint_Source", "Adjoint_Volume", "Fields_DFT", "Flux_Box", "Flux_Region",
+           "Geom_Info", "Simulation", "Symmetry", "direction_dict", "field_match_objective", "run_until_dft",
+           "run_until_fields_decay", "run_until_time", "side_dict"]
 
 direction_dict = {"x": 0, "y": 1, "z": 2}
 side_dict = {"positive": 1, "negative": -1}
+_AXES = "xyz"
 
 
-class _DatasetStore:
-    """Stands in for the input HDF5 file: objects' dump() writes named arrays."""
+def _vec_or_zero(v):
+    return v.copy() if v is not None else Vector3()
 
-    def __init__(self):
-        self.data = {}
 
-    def create_dataset(self, name, data):
-        self.data[name] = np.asarray(data, dtype=np.float64).ravel()
+# ------------------------------------------------------------ JSON specifications
+def _jsonable(value):
+    return value.get_json() if hasattr(value, "get_json") else value
+
+
+class _Spec:
+    """An object written as one JSON object: "type" (when the class has one)
+    and then one entry per (key, attribute) pair of ``json_keys``; attributes
+    that are None are left out when listed in ``json_optional``."""
+
+    json_type = None
+    json_keys = ()
+    json_optional = ()
+
+    def get_json(self):
+        out = {} if self.json_type is None else {"type": self.json_type}
+        for key, attr in self.json_keys:
+            value = getattr(self, attr)
+            if value is None and attr in self.json_optional:
+                continue
+            out[key] = _jsonable(value)
+        return out
+
+
+class run_until_time(_Spec):
+    """Stop once time_step * dt reaches `time` (src/simulation.cpp:768-780)."""
+
+    json_type = "time"
+    json_keys = (("time", "time"),)
+
+    def __init__(self, time=0):
+        self.time = float(time)
+
+
+class run_until_dft(_Spec):
+    """After the sources turn off, run windows of `time_interval_examined` until
+    the spread (max - min) / max of a registered DFT monitor's norm at
+    `frequency` over one window is at most `var` (src/simulation.cpp:782-842)."""
+
+    json_type = "dft"
+    json_keys = (("center", "center"), ("size", "size"), ("field component", "field_component"),
+                 ("time interval examined", "time_interval_examined"), ("variation", "var"),
+                 ("frequency", "frequency"))
+
+    def __init__(self, center=None, size=None, frequency=1, field_component="Ex", time_interval_examined=1,
+                 var=1e-2):
+        self.center, self.size = _vec_or_zero(center), _vec_or_zero(size)
+        self.frequency = float(frequency)
+        self.field_component = str(field_component)
+        self.time_interval_examined = float(time_interval_examined)
+        self.var = float(var)
+
+
+class run_until_fields_decay(_Spec):
+    """After the sources turn off, run windows of `time_interval_examined` until
+    a window's largest squared field at `position` is at most `decayed_by`
+    times the largest seen so far (src/simulation.cpp:844-901)."""
+
+    json_type = "decay"
+    json_keys = (("position", "position"), ("field component", "field_component"),
+                 ("time interval examined", "time_interval_examined"), ("decayed by", "decayed_by"))
+
+    def __init__(self, position=None, field_component="Ez", time_interval_examined=1, decayed_by=1e-3):
+        self.position = _vec_or_zero(position)
+        self.field_component = str(field_component)
+        self.time_interval_examined = float(time_interval_examined)
+        self.decayed_by = float(decayed_by)
+
+
+class Symmetry(_Spec):
+    """Mirror plane through the origin normal to `direction`: phase_factor +1
+    makes it a magnetic wall (PMC), -1 an electric wall (PEC); the grid keeps
+    the positive half (src/simulation.cpp:583-653)."""
+
+    json_keys = (("direction", "direction"), ("phase factor", "phase_factor"))
+
+    def __init__(self, direction="x", phase_factor=1):
+        self.direction = direction
+        self.phase_factor = int(phase_factor)
+
+
+class PML(_Spec):
+    """CPML layer of `thickness` on both sides of every axis, or on one
+    `direction` / `side`.  The grading is sigma_max (x/d)^order and
+    1 + (k_max - 1)(x/d)^order.  As in the reference's front end, a sigma_max
+    given here is not used: Simulation sets 0.8 (order + 1) resolution."""
+
+    json_keys = (("thickness", "thickness"), ("direction", "direction"), ("side", "side"),
+                 ("k max", "k_max"), ("order", "order"), ("sigma max", "sigma_max"))
+    json_optional = ("direction", "side")
+
+    def __init__(self, thickness, direction=None, side=None, k_max=1, order=3, sigma_max=None):
+        self.thickness = float(thickness)
+        self.direction, self.side = direction, side
+        self.k_max = float(k_max)
+        self.order = int(order)
+        self.sigma_max = None
+
+
+# ------------------------------------------------------------------ DFT monitors
+def _widened(freqs, axes, values, dx):
+    """Axes and values with every singleton axis doubled so that linear
+    interpolation works on it: a single frequency f becomes (0.9 f, 1.1 f), a
+    single coordinate c becomes (c - dx/2, c + dx/2), and the values are
+    repeated along that axis."""
+    freqs = freqs * np.array([0.9, 1.1]) if freqs.size == 1 else freqs
+    axes = [a + np.array([-dx / 2, dx / 2]) if a.size == 1 else a for a in axes]
+    for i in range(values.ndim):
+        if values.shape[i] == 1:
+            values = np.repeat(values, 2, axis=i)
+    return freqs, axes, values
 
 
 class Fields_DFT:
-    """One DFT output group (ffip/simulation.py:17-91)."""
+    """A volume DFT monitor of one field component: complex sums
+    sum_n F(t_n) exp(-2 pi i f t_n) at its frequencies over the points of its
+    box, read back as ``values`` [f, z, y, x] with the axes ``frequency``, ``z``,
+    ``y``, ``x``.  Output group "volume fields dft <suffix>"."""
 
     def __init__(self, center, size, frequency, field_component, suffix="0", degenerate=False, dx=0):
+        if not degenerate and dx == 0:
+            raise ValueError("a monitor read for interpolation needs the grid spacing dx")
         self.suffix = str(suffix)
-        self.group_name = "volume fields dft %s" % suffix
-        self.center = center.copy()
-        self.size = size.copy()
+        self.group_name = "volume fields dft %s" % self.suffix
+        self.center, self.size = center.copy(), size.copy()
         self.degenerate = bool(degenerate)
         self.dx = float(dx)
-        if not self.degenerate and dx == 0:
-            raise ValueError("for non degenerate dft, dx must be provided")
         self.frequency = np.sort(np.asarray(frequency, dtype=float))
         self.frequency_config = self.frequency.tolist()
         self.field_component = field_component
@@ -56,54 +180,58 @@ class Fields_DFT:
                 "frequency": self.frequency_config, "field component": self.field_component,
                 "output group": self.group_name}
 
+    def read(self, groups):
+        g = groups[self.group_name]
+        freqs = np.array(g["f"])
+        axes = [np.array(g[a]) for a in _AXES]
+        values = np.asarray(g["real"]) + 1j * np.asarray(g["imag"])
+        if not self.degenerate:
+            freqs, axes, values = _widened(freqs, axes, values, self.dx)
+        self.frequency = freqs
+        self.x, self.y, self.z = axes
+        self.v = values
+
     @property
     def values(self):
         return self.v
 
-    def read(self, groups):
-        g = groups[self.group_name]
-        self.frequency = np.array(g["f"])
-        self.x, self.y, self.z = np.array(g["x"]), np.array(g["y"]), np.array(g["z"])
-        self.v = g["real"] + 1j * g["imag"]
-        if not self.degenerate:
-            # singleton axes are widened so that linear interpolation works on them
-            if self.frequency.size == 1:
-                self.frequency = self.frequency * np.array([0.9, 1.1])
-            if self.x.size == 1:
-                self.x = self.x + np.array([-self.dx / 2, self.dx / 2])
-            if self.y.size == 1:
-                self.y = self.y + np.array([-self.dx / 2, self.dx / 2])
-            if self.z.size == 1:
-                self.z = self.z + np.array([-self.dx / 2, self.dx / 2])
-            for i in range(4):
-                if self.v.shape[i] == 1:
-                    self.v = np.concatenate((self.v, self.v), axis=i)
-
-    def __call__(self, f, z, y, x, substract=None, **kwargs):
-        interp = self.get_interpolant(substract, **kwargs)
-        return interp(np.stack(np.meshgrid(f, z, y, x, indexing="ij"), axis=-1))
-
-    def get_interpolant(self, substract=None, **kwargs):
-        if substract is not None and not isinstance(substract, Fields_DFT):
-            raise TypeError("Substract is not an instance of Fields_DFT")
-        vals = self.values - substract.values if substract is not None else self.values
-        return RegularGridInterpolator((self.frequency, self.z, self.y, self.x), vals, **kwargs)
-
     def get_metadata(self):
         return self.frequency, self.z, self.y, self.x
 
+    def get_interpolant(self, substract=None, **kwargs):
+        """Interpolant over (f, z, y, x) of the values, or of the values minus
+        those of another monitor (`substract`) with the same axes."""
+        if substract is not None and not isinstance(substract, Fields_DFT):
+            raise TypeError("substract must be a Fields_DFT, not %s" % type(substract).__name__)
+        data = self.values if substract is None else self.values - substract.values
+        return RegularGridInterpolator(self.get_metadata(), data, **kwargs)
+
+    def __call__(self, f, z, y, x, substract=None, **kwargs):
+        return self.get_interpolant(substract, **kwargs)(metadata_to_pts(f, z, y, x))
+
+
+# ------------------------------------------------------------------------- flux
+def _trapezoid(n):
+    w = np.ones(int(n))
+    if w.size > 1:
+        w[0] = w[-1] = 0.5
+    return w
+
 
 class Flux_Region:
-    """Poynting flux through a plane from an E and an H DFT (ffip/simulation.py:94-210)."""
+    """Poynting flux Re(E1 conj(H2)) through a plane normal to `direction`
+    (E1, H2 the components next in cyclic order), integrated with the
+    trapezoid rule on a lattice of spacing dx / scale and multiplied by
+    `weight`.  Registers one E and one H monitor on the plane."""
 
     def __init__(self, sim, center, size, frequency, weight):
-        dim = (size.x != 0) + (size.y != 0) + (size.z != 0)
-        if dim != 2:
-            raise NameError("Flux region dimension %d is invalid" % dim)
-        self.direction = 0 if size.x == 0 else 1 if size.y == 0 else 2
+        extended = [size[a] != 0 for a in range(3)]
+        if sum(extended) != 2:
+            raise NameError("a flux region must be a plane (two nonzero extents), got %d nonzero"
+                            % sum(extended))
+        self.direction = extended.index(False)
         self.weight = np.array(weight, copy=True)
-        self.center = center.copy()
-        self.size = size.copy()
+        self.center, self.size = center.copy(), size.copy()
         self.frequency = np.sort(np.asarray(frequency))
         self.dx = sim.dx
         self._e1_dft = sim.add_dft_fields(center=center, size=size, frequency=frequency, field_component=self.e1_str)
@@ -111,11 +239,11 @@ class Flux_Region:
 
     @property
     def e1_str(self):
-        return "E" + "xyz"[(self.direction + 1) % 3]
+        return "E" + _AXES[(self.direction + 1) % 3]
 
     @property
     def h2_str(self):
-        return "H" + "xyz"[(self.direction + 2) % 3]
+        return "H" + _AXES[(self.direction + 2) % 3]
 
     @property
     def e1_dft(self):
@@ -125,62 +253,64 @@ class Flux_Region:
     def h2_dft(self):
         return self._h2_dft
 
-    def get_fields(self, scale=1, substract=None, **kwargs):
-        if substract is not None and not isinstance(substract, Flux_Region):
-            raise TypeError("Substract is not an instance of Flux_Region")
-        e1 = self.e1_dft.get_interpolant(substract=substract.e1_dft if substract is not None else None, **kwargs)
-        h2 = self.h2_dft.get_interpolant(substract=substract.h2_dft if substract is not None else None, **kwargs)
-        pts = self.get_pts(scale=scale)
-        return e1(pts), h2(pts)
-
     def get_len(self, scale=1):
+        """Integration nodes per axis: size / dx * scale rounded, plus one."""
         return (self.size / self.dx * scale).round() + Vector3(1, 1, 1)
 
     def get_metadata(self, scale=1):
         n = self.get_len(scale=scale)
-        p1 = self.center - self.size / 2
-        p2 = self.center + self.size / 2
-        return (self.frequency, np.linspace(p1.z, p2.z, int(n.z)), np.linspace(p1.y, p2.y, int(n.y)),
-                np.linspace(p1.x, p2.x, int(n.x)))
+        lo, hi = self.center - self.size / 2, self.center + self.size / 2
+        xs = [np.linspace(lo[a], hi[a], int(n[a])) for a in range(3)]
+        return self.frequency, xs[2], xs[1], xs[0]
 
     def get_pts(self, scale=1):
-        return np.stack(np.meshgrid(*self.get_metadata(scale=scale), indexing="ij"), axis=-1)
+        return metadata_to_pts(*self.get_metadata(scale=scale))
 
     def get_int_weights(self, scale=1):
+        """Trapezoid weights [z, y, x] times the cell measure and `weight`."""
         n = self.get_len(scale=scale)
-        ws = [np.ones((int(n.x),)), np.ones((int(n.y),)), np.ones((int(n.z),))]
-        meas = 1
-        for a, w in zip("xyz", ws):
-            if w.size > 1:
-                meas *= getattr(self.size, a) / (w.size - 1)
-                w[0] = w[-1] = 0.5
-        Z, Y, X = np.meshgrid(ws[2], ws[1], ws[0], indexing="ij")
-        return X * Y * Z * meas * self.weight
+        measure = 1.0
+        for a in range(3):
+            if n[a] > 1:
+                measure *= self.size[a] / (n[a] - 1)
+        wz, wy, wx = (_trapezoid(n[a]) for a in (2, 1, 0))
+        return wz[:, None, None] * wy[None, :, None] * wx[None, None, :] * measure * self.weight
+
+    def get_fields(self, scale=1, substract=None, **kwargs):
+        """(E1, H2) on the integration lattice, [f, z, y, x] each."""
+        if substract is not None and not isinstance(substract, Flux_Region):
+            raise TypeError("substract must be a Flux_Region, not %s" % type(substract).__name__)
+        pts = self.get_pts(scale=scale)
+        fields = []
+        for mine, other in ((self.e1_dft, "e1_dft"), (self.h2_dft, "h2_dft")):
+            base = getattr(substract, other) if substract is not None else None
+            fields.append(mine.get_interpolant(substract=base, **kwargs)(pts))
+        return tuple(fields)
 
     def get_values(self, scale=1, substract=None, **kwargs):
+        """Flux at each frequency."""
         e1, h2 = self.get_fields(scale=scale, substract=substract, **kwargs)
-        flux = np.real(e1 * np.conj(h2)) * self.get_int_weights(scale=scale)
-        return np.sum(flux, axis=(1, 2, 3))
+        density = np.real(e1 * np.conj(h2)) * self.get_int_weights(scale=scale)
+        return density.sum(axis=(1, 2, 3))
 
     def get_interpolant(self, scale=1, substract=None):
         return RegularGridInterpolator((self.frequency,), self.get_values(scale=scale, substract=substract))
 
 
 class Flux_Box:
-    """Six signed flux planes around a box (ffip/simulation.py:212-250)."""
+    """Outward flux through the six faces of a box: flux regions ordered
+    x-, x+, y-, y+, z-, z+ with weights -1 and +1."""
 
     def __init__(self, sim, center, size, frequency):
-        self._flux_regions = []
         self.frequency = np.sort(np.asarray(frequency))
-        for d in range(3):
+        self._flux_regions = []
+        for a in range(3):
             for side in (-1, 1):
-                p1 = center - size / 2
-                p2 = center + size / 2
-                if side == -1:
-                    p2[d] = p1[d]
-                else:
-                    p1[d] = p2[d]
-                self._flux_regions.append(sim.add_flux_region(center=(p1 + p2) / 2, size=p2 - p1,
+                face_center = center.copy()
+                face_center[a] = center[a] + side * size[a] / 2
+                face_size = size.copy()
+                face_size[a] = 0.0
+                self._flux_regions.append(sim.add_flux_region(center=face_center, size=face_size,
                                                               frequency=frequency, weight=side))
 
     @property
@@ -188,22 +318,28 @@ class Flux_Box:
         return self._flux_regions
 
     def get_flux_region(self, direction="x", side="negative"):
-        return self.flux_regions[direction_dict[direction] * 2 + (0 if side_dict[side] == -1 else 1)]
+        return self._flux_regions[2 * direction_dict[direction] + (side_dict[side] > 0)]
 
     def get_values(self, scale=1, substract=None, **kwargs):
         if substract is not None and not isinstance(substract, Flux_Box):
-            raise TypeError("Substract is not an instance of Flux_Box")
-        return sum(self.flux_regions[i].get_values(
-            scale=scale, substract=substract.flux_regions[i] if substract is not None else None, **kwargs)
-            for i in range(6))
+            raise TypeError("substract must be a Flux_Box, not %s" % type(substract).__name__)
+        total = 0
+        for i, face in enumerate(self._flux_regions):
+            other = substract.flux_regions[i] if substract is not None else None
+            total = total + face.get_values(scale=scale, substract=other, **kwargs)
+        return total
 
     def get_interpolant(self, scale=1, substract=None, **kwargs):
         return RegularGridInterpolator((self.frequency,), self.get_values(scale=scale, substract=substract, **kwargs))
 
 
-class Geom_Info:
-    """Coordinates of the points of a component inside one geometry
-    (ffip/simulation.py:253-283, src/simulation.cpp:190-233)."""
+class Geom_Info(_Spec):
+    """Request for the coordinates of the points of one field component that a
+    geometry (by its index in the simulation's list) owns
+    (src/simulation.cpp:190-233).  Output group
+    "geometry info request <index> <component>"."""
+
+    json_keys = (("field component", "field_component"), ("geometry index", "index"), ("output group", "gname"))
 
     def __init__(self, index=0, field_component="Ex"):
         self.field_component = str(field_component)
@@ -212,10 +348,7 @@ class Geom_Info:
 
     def read(self, groups):
         g = groups[self.gname]
-        self._x, self._y, self._z = g["x"], g["y"], g["z"]
-
-    def get_json(self):
-        return {"field component": self.field_component, "geometry index": self.index, "output group": self.gname}
+        self._x, self._y, self._z = (np.asarray(g[a]) for a in _AXES)
 
     @property
     def x(self):
@@ -230,13 +363,33 @@ class Geom_Info:
         return self._z
 
 
+# ------------------------------------------------------------------- simulation
+class _Arrays:
+    """The input store: what objects' ``dump`` writes, by dataset name."""
+
+    def __init__(self):
+        self.data = {}
+
+    def create_dataset(self, name, data):
+        if name in self.data:
+            raise ValueError("two objects write the dataset %r" % name)
+        self.data[name] = np.asarray(data, dtype=np.float64).ravel()
+
+
+def _check_file_name(value):
+    if not isinstance(value, str):
+        raise ValueError("file names are strings, not %s" % type(value).__name__)
+    return value
+
+
 class Simulation:
-    """A simulation set-up; ``run`` executes it on the GPU (ffip/simulation.py:286-464).
+    """A simulation set-up.  ``run`` solves it on the GPU and reads the results
+    into its output objects; the engine of the last run stays available as
+    ``engine`` until the next run.
 
     Extra keyword ``precision``: "fp64" (default, the reference's arithmetic) or
-    "fp32" (fields and polarizations stored in FP32, FP64 arithmetic and DFT sums;
-    E/H monitors gather 16 steps in FP32 partial sums before adding into them).
-    """
+    "fp32" (fields and polarizations stored in FP32 with FP64 arithmetic,
+    coefficients and DFT sums)."""
 
     def __init__(self, size, resolution, pmls=[], geometry=[], sources=[], symmetry=[], default_material=Medium(),
                  periodic=False, progress_interval=4, courant=0.5, input_file="empty.h5",
@@ -244,248 +397,199 @@ class Simulation:
         print("Simulation created\n")
         self.size = size.copy()
         self.resolution = float(resolution)
-        self.geometry = deepcopy(geometry)
-        self.sources = deepcopy(sources)
-        self.symmetry = deepcopy(symmetry)
+        self.geometry = deepcopy(list(geometry))
+        self.sources = deepcopy(list(sources))
+        self.symmetry = deepcopy(list(symmetry))
         self.default_material = deepcopy(default_material)
         self.periodic = deepcopy(periodic)
         self.progress_interval = float(progress_interval)
         self.courant = float(courant)
-        self.pmls = deepcopy(pmls)
-        self.input_file = copy(input_file)
-        self.fields_output_file = copy(fields_output_file)
+        self.pmls = deepcopy(list(pmls))
+        self.input_file = input_file
+        self.fields_output_file = fields_output_file
         self.precision = precision
-        self.dft_fields = []
-        self.flux_regions = []
-        self.flux_boxes = []
-        self.geom_info_requests = []
+        self.dft_fields, self.flux_regions, self.flux_boxes, self.geom_info_requests = [], [], [], []
         self.engine = None
-        self._geom_key, self._geom_cache = None, {}
-        for p in self.pmls:
-            if p.sigma_max is None:
-                p.sigma_max = 0.8 * (1 + p.order) * resolution
+        self._geom_info_cache = (None, {})
+        for layer in self.pmls:
+            if layer.sigma_max is None:
+                layer.sigma_max = 0.8 * (1 + layer.order) * self.resolution
 
-    @property
-    def input_file(self):
-        return self._input_file
-
-    @input_file.setter
-    def input_file(self, val):
-        if not isinstance(val, str):
-            raise ValueError("file name is not a string")
-        self._input_file = val
-
-    @property
-    def fields_output_file(self):
-        return self._fields_output_file
-
-    @fields_output_file.setter
-    def fields_output_file(self, val):
-        if not isinstance(val, str):
-            raise ValueError("file name is not a string")
-        self._fields_output_file = val
+    input_file = property(lambda self: self._input_file,
+                          lambda self, v: setattr(self, "_input_file", _check_file_name(v)))
+    fields_output_file = property(lambda self: self._fields_output_file,
+                                  lambda self, v: setattr(self, "_fields_output_file", _check_file_name(v)))
 
     @property
     def dx(self):
         return 1 / self.resolution
 
+    # -- the document and its arrays
     def dump(self, store):
-        for g in self.geometry or []:
-            g.dump(store)
-        for s in self.sources or []:
-            s.dump(store)
+        for obj in list(self.geometry) + list(self.sources):
+            obj.dump(store)
 
     def get_json(self, stop_condition, decomposition=None):
-        res = {"courant number": self.courant, "resolution": self.resolution, "size": self.size.get_json(),
+        doc = {"courant number": self.courant, "resolution": self.resolution, "size": self.size.get_json(),
                "default material": self.default_material.get_json(), "progress interval": self.progress_interval,
                "input file": self.input_file, "fields output file": self.fields_output_file,
                "stop condition": stop_condition.get_json(), "precision": self.precision}
         if decomposition is not None:
-            res["decomposition"] = decomposition
-        if self.symmetry:
-            # the reference serialises self.periodic here (ffip/simulation.py:380),
-            # which fails for any symmetry list; the symmetry objects are meant
-            res["symmetry"] = [item.get_json() for item in self.symmetry]
-        if self.pmls:
-            res["PML"] = [item.get_json() for item in self.pmls]
+            doc["decomposition"] = decomposition
+        # (the reference's front end writes self.periodic under "symmetry",
+        # ffip/simulation.py:380, which cannot serialise; the symmetry planes go here)
+        optional = (("symmetry", self.symmetry), ("PML", self.pmls), ("geometry", self.geometry),
+                    ("sources", self.sources), ("fields output", self.dft_fields),
+                    ("geometry info request", self.geom_info_requests))
+        for key, items in optional:
+            if items:
+                doc[key] = [item.get_json() for item in items]
         if self.periodic:
-            res["periodic"] = self.periodic
-        if self.geometry:
-            res["geometry"] = [item.get_json() for item in self.geometry]
-        if self.sources:
-            res["sources"] = [item.get_json() for item in self.sources]
-        if self.dft_fields:
-            res["fields output"] = [item.get_json() for item in self.dft_fields]
-        if self.geom_info_requests:
-            res["geometry info request"] = [item.get_json() for item in self.geom_info_requests]
-        return res
+            doc["periodic"] = self.periodic
+        # keep the reference's key order for the documents a user inspects
+        order = ["courant number", "resolution", "size", "default material", "progress interval", "input file",
+                 "fields output file", "stop condition", "precision", "decomposition", "symmetry", "PML",
+                 "periodic", "geometry", "sources", "fields output", "geometry info request"]
+        return {k: doc[k] for k in order if k in doc}
 
+    def _inputs(self):
+        store = _Arrays()
+        self.dump(store)
+        return store.data
+
+    # -- running
     def run(self, stop_condition, decomposition=None, np=1, skip=False, pop=False, **kwargs):
-        """Run to the stop condition and read the DFT and geometry-info results
-        into the registered objects.  ``skip=True`` re-reads the previous run."""
+        """Solve to the stop condition and read the results into the output
+        objects.  ``skip=True`` re-reads the last run's results without solving."""
         if not skip or self.engine is None:
-            config = self.get_json(stop_condition=stop_condition, decomposition=decomposition)
-            store = _DatasetStore()
-            self.dump(store)
+            doc = self.get_json(stop_condition=stop_condition, decomposition=decomposition)
             if self.engine is not None:
-                # the previous run's results have been read; releasing its device
-                # memory first lets the new engine reuse the blocks
+                # free the previous engine's device memory before the new one allocates
                 self.engine.close()
                 self.engine = None
-            self.engine = LibSim(config, store.data)
-            self.engine.run(config["stop condition"])
-            key = json.dumps(config, sort_keys=True)
-            if key != self._geom_key:
-                self._geom_key, self._geom_cache = key, {}
+            self.engine = LibSim(doc, self._inputs())
+            self.engine.run(doc["stop condition"])
+            self._last_doc = doc
+        self._read_results()
+
+    def _read_results(self):
         groups = {}
         for k in range(self.engine.dft_count()):
-            d = self.engine.dft(k)
-            groups[d["group"]] = d
-        for k, req in enumerate(self.geom_info_requests):
-            # which points a geometry owns depends on the document only (shapes and
-            # grid, not the datasets), so an unchanged document reuses them
-            if k not in self._geom_cache:
-                self._geom_cache[k] = self.engine.geom_info(k)
-            x, y, z = self._geom_cache[k]
-            groups[req.gname] = {"x": x, "y": y, "z": z}
-        for item in self.dft_fields:
-            item.read(groups)
-        for item in self.geom_info_requests:
+            out = self.engine.dft(k)
+            groups[out["group"]] = out
+        groups.update(self._geometry_info())
+        for item in list(self.dft_fields) + list(self.geom_info_requests):
             item.read(groups)
 
+    def _geometry_info(self):
+        # which points a geometry owns depends on the document only (shapes and
+        # grid, not the arrays), so an unchanged document reuses the answer
+        key = repr(sorted(self._last_doc.items())) if self.geom_info_requests else None
+        cached_key, cache = self._geom_info_cache
+        if key != cached_key:
+            cache = {}
+            self._geom_info_cache = (key, cache)
+        for k, req in enumerate(self.geom_info_requests):
+            if k not in cache:
+                cache[k] = dict(zip(_AXES, self.engine.geom_info(k)))
+        return {req.gname: cache[k] for k, req in enumerate(self.geom_info_requests)}
+
+    # -- building up
     def add_geometry(self, geom):
+        """Insert `geom` first: it then wins over every geometry given before."""
         self.geometry.insert(0, geom)
 
     def add_source(self, src):
         self.sources.insert(0, src)
 
     def request_geom_info(self, geom, field_component="Ex"):
-        for i in range(len(self.geometry)):
-            if self.geometry[i] is geom:
+        for i, g in enumerate(self.geometry):
+            if g is geom:
                 req = Geom_Info(index=i, field_component=field_component)
                 self.geom_info_requests.append(req)
                 return req
-        raise ValueError("Geometry non-existed")
+        raise ValueError("the geometry is not part of this simulation")
 
-    def add_dft_fields(self, center=Vector3(), size=Vector3(), frequency=[], field_component="Ex"):
-        res = Fields_DFT(center, size, frequency, field_component, str(len(self.dft_fields)), degenerate=False,
-                         dx=self.dx)
-        self.dft_fields.append(res)
-        return res
+    def add_dft_fields(self, center=None, size=None, frequency=[], field_component="Ex"):
+        mon = Fields_DFT(_vec_or_zero(center), _vec_or_zero(size), frequency, field_component,
+                         str(len(self.dft_fields)), degenerate=False, dx=self.dx)
+        self.dft_fields.append(mon)
+        return mon
 
-    def add_flux_region(self, center=Vector3(), size=Vector3(), frequency=[], weight=1):
-        res = Flux_Region(self, center=center, size=size, frequency=frequency, weight=weight)
-        self.flux_regions.append(res)
-        return res
+    def add_flux_region(self, center=None, size=None, frequency=[], weight=1):
+        region = Flux_Region(self, center=_vec_or_zero(center), size=_vec_or_zero(size), frequency=frequency,
+                             weight=weight)
+        self.flux_regions.append(region)
+        return region
 
-    def add_flux_box(self, center=Vector3(), size=Vector3(), frequency=[]):
-        res = Flux_Box(self, center=center, size=size, frequency=frequency)
-        self.flux_boxes.append(res)
-        return res
-
-
-class Symmetry:
-    def __init__(self, direction="x", phase_factor=1):
-        self.direction = direction
-        self.phase_factor = int(phase_factor)
-
-    def get_json(self):
-        return {"direction": self.direction, "phase factor": self.phase_factor}
+    def add_flux_box(self, center=None, size=None, frequency=[]):
+        box = Flux_Box(self, center=_vec_or_zero(center), size=_vec_or_zero(size), frequency=frequency)
+        self.flux_boxes.append(box)
+        return box
 
 
-class run_until_time:
-    def __init__(self, time=0):
-        self.time = float(time)
+# --------------------------------------------------------------- objectives
+class _ComplexMatch:
+    """F = sum w |E - E0|^2 / 2, conj(dF/dE) = w conj(E - E0)."""
 
-    def get_json(self):
-        return {"type": "time", "time": self.time}
+    def __init__(self, target, weight):
+        self.target, self.weight = target, weight
 
+    def fun(self, e):
+        return np.sum(0.5 * np.ravel(self.weight) * np.abs(np.ravel(e) - np.ravel(self.target)) ** 2)
 
-class run_until_dft:
-    def __init__(self, center=Vector3(), size=Vector3(), frequency=1, field_component="Ex",
-                 time_interval_examined=1, var=1e-2):
-        self.center, self.size = center.copy(), size.copy()
-        self.field_component = str(field_component)
-        self.time_interval_examined = float(time_interval_examined)
-        self.var = float(var)
-        self.frequency = float(frequency)
-
-    def get_json(self):
-        return {"type": "dft", "center": self.center.get_json(), "size": self.size.get_json(),
-                "field component": self.field_component, "time interval examined": self.time_interval_examined,
-                "variation": self.var, "frequency": self.frequency}
+    def fprime(self, e):
+        return (np.conj(e - self.target) * self.weight,)
 
 
-class run_until_fields_decay:
-    def __init__(self, position=Vector3(), field_component="Ez", time_interval_examined=1, decayed_by=1e-3):
-        self.position = position.copy()
-        self.field_component = str(field_component)
-        self.time_interval_examined = float(time_interval_examined)
-        self.decayed_by = float(decayed_by)
+class _ModulusMatch(_ComplexMatch):
+    """F = sum w (|E| - A0)^2 / 2, conj(dF/dE) = w (1 - A0 / |E|) conj(E)."""
 
-    def get_json(self):
-        return {"type": "decay", "position": self.position.get_json(), "field component": self.field_component,
-                "time interval examined": self.time_interval_examined, "decayed by": self.decayed_by}
+    def fun(self, e):
+        return np.sum(0.5 * np.ravel(self.weight) * (np.abs(np.ravel(e)) - np.ravel(self.target)) ** 2)
+
+    def fprime(self, e):
+        return ((1 - self.target / np.abs(e)) * np.conj(e) * self.weight,)
 
 
-class PML:
-    """CPML layer.  As in the reference (ffip/simulation.py:519-542) the sigma_max
-    argument is not kept: Simulation sets 0.8 (order + 1) resolution."""
+class _VectorModulusMatch(_ComplexMatch):
+    """F = sum w (|E| - A0)^2 / 2 with |E| the length of (Ex, Ey, Ez)."""
 
-    def __init__(self, thickness, direction=None, side=None, k_max=1, order=3, sigma_max=None):
-        self.thickness = float(thickness)
-        self.direction, self.side = direction, side
-        self.k_max = float(k_max)
-        self.order = int(order)
-        self.sigma_max = None
+    @staticmethod
+    def _length(ex, ey, ez):
+        return np.sqrt(np.abs(ex) ** 2 + np.abs(ey) ** 2 + np.abs(ez) ** 2)
 
-    def get_json(self):
-        res = {"thickness": self.thickness}
-        if self.direction is not None:
-            res["direction"] = self.direction
-        if self.side is not None:
-            res["side"] = self.side
-        res["k max"] = self.k_max if self.k_max is not None else 1.0
-        res["order"] = self.order if self.order is not None else 3
-        res["sigma max"] = self.sigma_max
-        return res
+    def fun(self, ex, ey, ez):
+        r = self._length(ex, ey, ez)
+        return np.sum(0.5 * (np.ravel(r) - np.ravel(self.target)) ** 2 * np.ravel(self.weight))
+
+    def fprime(self, ex, ey, ez):
+        factor = (1 - self.target / self._length(ex, ey, ez)) * self.weight
+        return tuple(factor * np.conj(c) for c in (ex, ey, ez))
 
 
 def field_match_objective(obj_val, component="Ex", weight=1):
-    """(components, conj(dF/dE) function, F function) of a field-matching
-    objective (ffip/simulation.py:544-582)."""
-    if component in ["Ex", "Ey", "Ez"]:
-        def fun1(val):
-            return np.sum(0.5 * np.ravel(weight) * np.abs(val.ravel() - obj_val.ravel()) ** 2)
-
-        def fpconj1(val):
-            return (np.conj(val - obj_val) * weight,)
-
-        return [component], fpconj1, fun1
-    if component in ["|Ex|", "|Ey|", "|Ez|"]:
-        def fun2(val):
-            return np.sum(0.5 * np.ravel(weight) * (np.abs(val).ravel() - obj_val.ravel()) ** 2)
-
-        def fpconj2(val):
-            return ((1 - obj_val / np.abs(val)) * np.conj(val) * weight,)
-
-        return [component[1:-1]], fpconj2, fun2
-    if component == "|E|":
-        def fun3(ex, ey, ez):
-            val = np.sqrt(np.abs(ex) ** 2 + np.abs(ey) ** 2 + np.abs(ez) ** 2)
-            return np.sum(0.5 * (np.ravel(val) - np.ravel(obj_val)) ** 2 * np.ravel(weight))
-
-        def fpconj3(ex, ey, ez):
-            val = np.sqrt(np.abs(ex) ** 2 + np.abs(ey) ** 2 + np.abs(ez) ** 2)
-            r = (1 - obj_val / val) * weight
-            return (r * np.conj(ex), r * np.conj(ey), r * np.conj(ez))
-
-        return ["Ex", "Ey", "Ez"], fpconj3, fun3
-    raise ValueError("unknown objective component %r" % component)
+    """(components, fprime, fun) of a field-matching objective: `fun` maps the
+    components' DFT values to F and `fprime` to conj(dF/dE) per component, the
+    adjoint source amplitudes.  `component`: "Ex", "Ey", "Ez" (complex match),
+    "|Ex|", "|Ey|", "|Ez|" (modulus match) or "|E|" (vector length)."""
+    if component in ("Ex", "Ey", "Ez"):
+        m, comps = _ComplexMatch(obj_val, weight), [component]
+    elif component in ("|Ex|", "|Ey|", "|Ez|"):
+        m, comps = _ModulusMatch(obj_val, weight), [component[1:3]]
+    elif component == "|E|":
+        m, comps = _VectorModulusMatch(obj_val, weight), ["Ex", "Ey", "Ez"]
+    else:
+        raise ValueError("objective component %r is not one of Ex, Ey, Ez, |Ex|, |Ey|, |Ez|, |E|" % (component,))
+    return comps, m.fprime, m.fun
 
 
+# ------------------------------------------------------------- adjoint sources
 class Adjoint_Flux:
-    """Adjoint sources of a flux objective (ffip/simulation.py:585-645)."""
+    """Adjoint sources of a flux objective F = sum of the fluxes' values: on
+    each flux plane, an E1 source of amplitude conj(H2) w and an H2 source of
+    amplitude -conj(E1) w (w the integration weights), on the plane's
+    integration lattice."""
 
     def __init__(self, adjoint_simulation, forward_simulation, function, fluxes=[], frequency=1.0, scale=1):
         self.adjoint_simulation = adjoint_simulation
@@ -496,162 +600,170 @@ class Adjoint_Flux:
         self.e_adjoint_sources, self.m_adjoint_sources = [], []
         for i, flux in enumerate(fluxes):
             if flux.frequency.size > 1:
-                raise ValueError("Fluxes used in Adjoint_Flux cannot have more than 1 frequency")
-            for lst, comp in ((self.e_adjoint_sources, flux.e1_str), (self.m_adjoint_sources, flux.h2_str)):
-                lst.append(Inhom_Source(function=function, frequency=frequency, amplitude=None, center=flux.center,
-                                        size=flux.size, dim=flux.get_len(scale=scale), field_component=comp,
-                                        suffix="flux %d %s adjoint" % (i, comp)))
-            adjoint_simulation.add_source(self.e_adjoint_sources[-1])
-            adjoint_simulation.add_source(self.m_adjoint_sources[-1])
+                raise ValueError("a flux in an adjoint flux objective must have one frequency, flux %d has %d"
+                                 % (i, flux.frequency.size))
+            for group, comp in ((self.e_adjoint_sources, flux.e1_str), (self.m_adjoint_sources, flux.h2_str)):
+                src = Inhom_Source(function=function, frequency=frequency, amplitude=None, center=flux.center,
+                                   size=flux.size, dim=flux.get_len(scale=scale), field_component=comp,
+                                   suffix="flux %d %s adjoint" % (i, comp))
+                group.append(src)
+                adjoint_simulation.add_source(src)
 
     def eval_functionals_and_set_sources(self):
-        res = 0
-        for i, flux in enumerate(self.fluxes):
-            res += flux.get_values(scale=self.scale)
+        """F from the forward run's monitors; sets the adjoint amplitudes."""
+        total = 0
+        for flux, e_src, m_src in zip(self.fluxes, self.e_adjoint_sources, self.m_adjoint_sources):
+            total = total + flux.get_values(scale=self.scale)
             e1, h2 = flux.get_fields(scale=self.scale)
             w = flux.get_int_weights(scale=self.scale)
-            self.e_adjoint_sources[i].amplitude = np.conj(h2[0, ...]) * w
-            self.m_adjoint_sources[i].amplitude = -np.conj(e1[0, ...]) * w
-        return res
+            e_src.amplitude = np.conj(h2[0]) * w
+            m_src.amplitude = -np.conj(e1[0]) * w
+        return total
 
 
 class Adjoint_Source:
-    """Objective on a patch of forward DFT fields and the matching adjoint
-    sources (ffip/simulation.py:648-757)."""
+    """Objective on the forward DFT fields over a patch (`dim` nodes of a box)
+    and the matching adjoint sources.  `functionals`: [component, target]
+    pairs (field matches); `udfs`: (components, fprime, fun) triples such as
+    ``field_match_objective`` returns."""
 
     comp_map = {"Ex": ["Ex"], "Ey": ["Ey"], "Ez": ["Ez"], "|Ex|": ["Ex"], "|Ey|": ["Ey"], "|Ez|": ["Ez"],
                 "|E|": ["Ex", "Ey", "Ez"]}
 
-    def __init__(self, adjoint_simulation, forward_simulation, function, frequency=1.0, center=Vector3(),
-                 size=Vector3(), dim=Vector3(), functionals=[], udfs=[]):
+    def __init__(self, adjoint_simulation, forward_simulation, function, frequency=1.0, center=None, size=None,
+                 dim=None, functionals=[], udfs=[]):
         self.adjoint_simulation = adjoint_simulation
         self.forward_simulation = forward_simulation
         self.frequency = float(frequency)
-        self.center, self.size = center.copy(), size.copy()
-        self.dim = dim.round()
+        self.center, self.size = _vec_or_zero(center), _vec_or_zero(size)
+        self.dim = _vec_or_zero(dim).round()
+        self.function = deepcopy(function)
         self.functionals = deepcopy(functionals)
         self.udfs = list(udfs)
-        self.function = deepcopy(function)
         self.forward_dfts, self.adjoint_sources, self.forward_fields = {}, {}, {}
-        p1 = self.center - self.size / 2
-        self.x = np.linspace(p1.x, p1.x + self.size.x, int(dim.x))
-        self.y = np.linspace(p1.y, p1.y + self.size.y, int(dim.y))
-        self.z = np.linspace(p1.z, p1.z + self.size.z, int(dim.z))
-        for func in functionals:
-            if not cmp_shape(self.shape, func[1].shape):
-                raise ValueError("objective values does not match the shape")
-            self.udfs.append(field_match_objective(func[1], func[0], weight=1))
-        for components, fprime, fun in self.udfs:
-            if not callable(fprime):
-                raise ValueError("The derivative function is not callable")
-            if not callable(fun):
-                raise ValueError("The objective function is not callable")
-            for c in components:
-                self.add_component(c)
+        lo = self.center - self.size / 2
+        self.x, self.y, self.z = (np.linspace(lo[a], lo[a] + self.size[a], int(self.dim[a])) for a in range(3))
+        for comp, target in functionals:
+            if not cmp_shape(self.shape, np.shape(target)):
+                raise ValueError("objective target of shape %s does not match the patch %s"
+                                 % (np.shape(target), self.shape))
+            self.udfs.append(field_match_objective(target, comp, weight=1))
+        for comps, fprime, fun in self.udfs:
+            if not callable(fprime) or not callable(fun):
+                raise ValueError("an objective needs callable fun and fprime")
+            for comp in comps:
+                self.add_component(comp)
 
     def add_component(self, component="Ex"):
-        if component not in self.forward_dfts:
-            self.forward_dfts[component] = self.forward_simulation.add_dft_fields(
-                center=self.center, size=self.size, frequency=[self.frequency], field_component=component)
-            self.adjoint_sources[component] = Inhom_Source(
-                function=self.function, frequency=self.frequency, amplitude=None, center=self.center, size=self.size,
-                dim=self.dim, field_component=component, suffix=component + " adjoint")
-            self.adjoint_simulation.add_source(self.adjoint_sources[component])
+        """A forward monitor and an adjoint source of `component` on the patch."""
+        if component in self.forward_dfts:
+            return
+        self.forward_dfts[component] = self.forward_simulation.add_dft_fields(
+            center=self.center, size=self.size, frequency=[self.frequency], field_component=component)
+        src = Inhom_Source(function=self.function, frequency=self.frequency, amplitude=None, center=self.center,
+                           size=self.size, dim=self.dim, field_component=component, suffix=component + " adjoint")
+        self.adjoint_sources[component] = src
+        self.adjoint_simulation.add_source(src)
 
     @property
     def shape(self):
         return (int(self.dim.z), int(self.dim.y), int(self.dim.x))
 
-    def eval_functionals_and_set_sources(self):
-        for c in self.forward_dfts:
-            self.adjoint_sources[c].amplitude = np.zeros(self.shape, dtype=complex)
-            self.forward_fields[c] = self.forward_dfts[c](self.frequency, self.z, self.y, self.x).reshape(self.shape)
-        res = 0
-        for components, fprime, fun in self.udfs:
-            args = [self.forward_fields[c] for c in components]
-            res = res + fun(*args)
-            der = fprime(*args)
-            for i, c in enumerate(components):
-                self.adjoint_sources[c].amplitude += der[i]
-        return res
-
     @property
     def numel(self):
         return int(self.dim.prod())
 
+    def eval_functionals_and_set_sources(self):
+        """F from the forward run's monitors; sets the adjoint amplitudes to
+        the sum over objectives of conj(dF/dE)."""
+        for comp, mon in self.forward_dfts.items():
+            self.forward_fields[comp] = mon(self.frequency, self.z, self.y, self.x).reshape(self.shape)
+        amps = {comp: np.zeros(self.shape, dtype=complex) for comp in self.forward_dfts}
+        total = 0
+        for comps, fprime, fun in self.udfs:
+            args = [self.forward_fields[c] for c in comps]
+            total = total + fun(*args)
+            for c, d in zip(comps, fprime(*args)):
+                amps[c] = amps[c] + d
+        for comp, a in amps.items():
+            self.adjoint_sources[comp].amplitude = a
+        return total
+
 
 class Adjoint_Volume:
-    """Design volume: forward/adjoint DFTs over it and the sensitivity
-    Re(2 pi i f E_fwd E_adj eps'(rho) / norm) mapped onto the density nodes
-    (ffip/simulation.py:759-915).  Only the two-medium (linear) design is
-    provided."""
+    """A two-medium design box shared by a forward and an adjoint simulation,
+    E monitors over it in both, and the sensitivity
+        dF/deps = Re(2 pi i f E_fwd E_adj eps'(rho) / norm)
+    mapped onto the density nodes with the adjoint of trilinear interpolation.
+    Only the linear two-medium design (epsilon_fun=None) is provided."""
 
-    def __init__(self, adjoint_simulation, forward_simulation, frequency=1.0, center=Vector3(), size=Vector3(),
-                 dim=Vector3(), density=None, medium1=Medium(), medium2=Medium(), epsilon_fun=None,
-                 epsilon_der=None, e_sus=[], norm=1):
+    _COMPS = ("Ex", "Ey", "Ez")
+
+    def __init__(self, adjoint_simulation, forward_simulation, frequency=1.0, center=None, size=None, dim=None,
+                 density=None, medium1=Medium(), medium2=Medium(), epsilon_fun=None, epsilon_der=None, e_sus=[],
+                 norm=1):
         if epsilon_fun is not None:
-            raise NotImplementedError("only the two-medium design (epsilon_fun=None) is supported")
+            raise NotImplementedError("only the two-medium design (epsilon_fun=None) is provided")
         self.adjoint_simulation = adjoint_simulation
         self.forward_simulation = forward_simulation
         self.frequency = float(frequency)
-        self.center, self.size = center.copy(), size.copy()
-        self.dim = dim.round()
+        self.center, self.size = _vec_or_zero(center), _vec_or_zero(size)
+        self.dim = _vec_or_zero(dim).round()
         self.norm = complex(norm)
-        e1 = medium1.get_epsilon(self.frequency)
-        e2 = medium2.get_epsilon(self.frequency)
+        self.medium1, self.medium2 = deepcopy(medium1), deepcopy(medium2)
+        e1, e2 = medium1.get_epsilon(self.frequency), medium2.get_epsilon(self.frequency)
         self.epsilon_fun = lambda rho: e1 * rho + (1 - rho) * e2
         self.epsilon_der = lambda rho: e1 - e2
-        self.medium1, self.medium2 = deepcopy(medium1), deepcopy(medium2)
-        self.geom = Two_Medium_Box(size=size, center=center, dim=dim, density=density, medium1=medium1,
-                                   medium2=medium2, suffix="adjoint")
-        adjoint_simulation.add_geometry(self.geom)
-        forward_simulation.add_geometry(self.geom)
-        comps = ["Ex", "Ey", "Ez"]
-        self.forward_dfts = [forward_simulation.add_dft_fields(center=center, size=size, frequency=[frequency],
-                                                               field_component=c) for c in comps]
-        self.adjoint_dfts = [adjoint_simulation.add_dft_fields(center=center, size=size, frequency=[frequency],
-                                                               field_component=c) for c in comps]
-        self.geom_infos = [forward_simulation.request_geom_info(geom=self.geom, field_component=c) for c in comps]
+        self.geom = Two_Medium_Box(size=self.size, center=self.center, dim=self.dim, density=density,
+                                   medium1=medium1, medium2=medium2, suffix="adjoint")
+        for sim in (adjoint_simulation, forward_simulation):
+            sim.add_geometry(self.geom)
+        self.forward_dfts = [forward_simulation.add_dft_fields(center=self.center, size=self.size,
+                                                               frequency=[self.frequency], field_component=c)
+                             for c in self._COMPS]
+        self.adjoint_dfts = [adjoint_simulation.add_dft_fields(center=self.center, size=self.size,
+                                                               frequency=[self.frequency], field_component=c)
+                             for c in self._COMPS]
+        self.geom_infos = [forward_simulation.request_geom_info(geom=self.geom, field_component=c)
+                           for c in self._COMPS]
 
     def forward_dft(self, field_component="Ex"):
-        return self.forward_dfts["xyz".index(field_component[1])]
+        return self.forward_dfts[self._COMPS.index(field_component)]
 
     def adjoint_dft(self, field_component="Ex"):
-        return self.adjoint_dfts["xyz".index(field_component[1])]
+        return self.adjoint_dfts[self._COMPS.index(field_component)]
 
     def get_sensitivity(self):
-        """Sensitivity sampled at the E points inside the design box and
-        scattered onto the density nodes with the adjoint of trilinear
-        interpolation.  The points are those of the geometry-info requests,
-        whose coordinates the reference truncates to integers
-        (src/simulation.cpp:201-211); they are used as given."""
-        pts = [np.stack((self.frequency * np.ones(g.z.shape), g.z, g.y, g.x), axis=-1) for g in self.geom_infos]
-        rho_interp = RegularGridInterpolator((self.z, self.y, self.x), self.density, bounds_error=True)
-        self.rho = [rho_interp(np.stack((g.z, g.y, g.x), axis=-1)) for g in self.geom_infos]
-        self.forward_fields = [self.forward_dfts[i].get_interpolant(method="linear", bounds_error=True)(pts[i])
-                               for i in range(3)]
-        self.adjoint_fields = [self.adjoint_dfts[i].get_interpolant(method="linear", bounds_error=True)(pts[i])
-                               for i in range(3)]
-        self.se, self.se_transposed = [], []
-        res = 0
-        for i in range(3):
-            self.se.append(np.real(2j * pi * self.frequency * self.forward_fields[i] * self.adjoint_fields[i] *
-                                   self.epsilon_der(self.rho[i]) / self.norm))
-            g = self.geom_infos[i]
-            self.se_transposed.append(transpose(self.x, self.y, self.z, np.stack((g.x, g.y, g.z, self.se[i]), axis=-1)))
-            res = res + self.se_transposed[i]
-        return np.reshape(res, self.shape)
+        """Sensitivity at the E points the design box owns (their coordinates
+        from the geometry-info requests, which the reference truncates to
+        integers, src/simulation.cpp:201-211, and which are used as given),
+        scattered onto the density nodes."""
+        rho_at = RegularGridInterpolator((self.z, self.y, self.x), self.density, bounds_error=True)
+        self.rho, self.forward_fields, self.adjoint_fields, self.se, self.se_transposed = [], [], [], [], []
+        total = 0
+        for fwd, adj, info in zip(self.forward_dfts, self.adjoint_dfts, self.geom_infos):
+            pts = np.stack((np.full(info.z.shape, self.frequency), info.z, info.y, info.x), axis=-1)
+            rho = rho_at(pts[:, 1:])
+            ef = fwd.get_interpolant(method="linear", bounds_error=True)(pts)
+            ea = adj.get_interpolant(method="linear", bounds_error=True)(pts)
+            se = np.real(2j * pi * self.frequency * ef * ea * self.epsilon_der(rho) / self.norm)
+            spread = transpose(self.x, self.y, self.z, np.stack((info.x, info.y, info.z, se), axis=-1))
+            for lst, v in ((self.rho, rho), (self.forward_fields, ef), (self.adjoint_fields, ea), (self.se, se),
+                           (self.se_transposed, spread)):
+                lst.append(v)
+            total = total + spread
+        return np.reshape(total, self.shape)
 
     def get_sensitivity_gpu(self, norm=None, factor=1.0):
-        """get_sensitivity computed on the GPU from the two simulations' device
-        DFT sums (ffip_adjoint_gradient): the E points are taken at their exact
-        positions and the transpose is a deterministic gather.  ``norm`` defaults
-        to the norm given at construction; ``factor`` multiplies eps' (used to
-        replace 2 pi i f by the time-discrete derivative factor)."""
+        """get_sensitivity formed on the GPU from the two simulations' device
+        DFT sums (ffip_adjoint_gradient): the E points at their exact positions
+        and the transpose as a deterministic gather.  ``norm`` defaults to the
+        construction's; ``factor`` multiplies eps' (it replaces 2 pi i f by
+        the time-discrete derivative)."""
         from ._lib import adjoint_gradient
         fe, ae = self.forward_simulation.engine, self.adjoint_simulation.engine
         if fe is None or ae is None:
-            raise RuntimeError("run the forward and adjoint simulations first")
+            raise RuntimeError("run the forward and the adjoint simulation first")
         fwd_out = [self.forward_simulation.dft_fields.index(d) for d in self.forward_dfts]
         adj_out = [self.adjoint_simulation.dft_fields.index(d) for d in self.adjoint_dfts]
         eps_der = complex(self.epsilon_der(0.0)) * complex(factor)
@@ -659,13 +771,14 @@ class Adjoint_Volume:
                                 complex(self.norm if norm is None else norm), self.center, self.size, self.dim)
 
     def get_sensitivity2(self):
-        pts = np.stack(np.meshgrid(self.frequency, self.z, self.y, self.x, indexing="ij"), axis=-1)
-        res = 0
-        for i in range(3):
-            f = self.forward_dfts[i].get_interpolant(method="linear", bounds_error=True)(pts)
-            a = self.adjoint_dfts[i].get_interpolant(method="linear", bounds_error=True)(pts)
-            res = res + 2j * pi * self.frequency * f * a / self.norm
-        return np.real(np.reshape(res, self.shape) * self.epsilon_der(self.density))
+        """The sensitivity evaluated at the density nodes themselves."""
+        pts = metadata_to_pts(self.frequency, self.z, self.y, self.x)
+        acc = 0
+        for fwd, adj in zip(self.forward_dfts, self.adjoint_dfts):
+            ef = fwd.get_interpolant(method="linear", bounds_error=True)(pts)
+            ea = adj.get_interpolant(method="linear", bounds_error=True)(pts)
+            acc = acc + 2j * pi * self.frequency * ef * ea / self.norm
+        return np.real(np.reshape(acc, self.shape) * self.epsilon_der(self.density))
 
     @property
     def x(self):
@@ -688,8 +801,8 @@ class Adjoint_Volume:
         return self.geom.density
 
     @density.setter
-    def density(self, val):
-        self.geom.density = val
+    def density(self, value):
+        self.geom.density = value
 
     @property
     def numel(self):

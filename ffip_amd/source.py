@@ -1,54 +1,78 @@
-"""Sources of the front end (the public names of ffip/source.py)."""
+"""Excitations of the front end.
+
+The names, constructor arguments, attributes, JSON keys and dataset names are
+those of ffip/source.py, which scripts use; the JSON is what
+Simulation::set_volume_source / set_inhom_source read
+(src/simulation.cpp:235-336).  An inhomogeneous source is a sampled array on a
+box and shares the node layout of the density boxes (``geom.NodeGrid``).
+"""
 from __future__ import annotations
 
 from copy import deepcopy
 
 import numpy as np
 
-from .geom import Vector3, cmp_shape
+from .geom import NodeGrid, Vector3
+
+__all__ = ["Gaussian1", "Inhom_Source", "Source", "Source_Function"]
 
 
 class Source_Function:
+    """A pulse active on (start_time, end_time)."""
+
+    start_time = 0.0
+    end_time = 0.0
+
     @property
     def duration(self):
         return self.end_time - self.start_time
 
 
 class Gaussian1(Source_Function):
-    """First derivative of a Gaussian, width 1/(2 pi f), active on
-    (start, start + 2 * cutoff * width) (ffip/source.py:31-53,
-    src/utility.cpp:260-264)."""
+    """First derivative of a Gaussian, g(t) = u exp(-u^2 / 2) with
+    u = (t - t_c) / width, width = 1 / (2 pi frequency), active for
+    2 cutoff widths from `start_time` and centred in that window
+    (src/utility.cpp:260-264, src/fields.cpp:17-33)."""
 
     def __init__(self, frequency, cutoff=4, start_time=0):
         self.frequency = float(frequency)
         self.cutoff = float(cutoff)
         self.start_time = float(start_time)
-        self.width = 1 / (2 * np.pi * frequency)
-        self.end_time = self.start_time + self.width * cutoff * 2
+        # evaluated in the engine's operation order, so that the window ends
+        # agree to the last bit
+        self.width = 1 / (2 * np.pi * self.frequency)
+        self.end_time = self.start_time + self.width * self.cutoff * 2
         self.offset_time = (self.start_time + self.end_time) / 2
+
+    def __call__(self, t):
+        u = (np.asarray(t) - self.offset_time) / self.width
+        return u * np.exp(-0.5 * u * u)
 
     def get_json(self):
         return {"type": "Gaussian1", "frequency": self.frequency, "cutoff": self.cutoff,
                 "start time": self.start_time}
 
-    def __call__(self, t):
-        arg = (t - self.offset_time) / self.width
-        return arg * np.exp(-(arg ** 2) / 2)
+
+def _field_name(component):
+    name = str(component)
+    if len(name) != 2 or name[0] not in "EH" or name[1] not in "xyz":
+        raise ValueError("field component must be one of Ex, Ey, Ez, Hx, Hy, Hz, not %r" % (component,))
+    return name
 
 
 class Source:
-    """Volume source of dipoles spaced by dx (ffip/source.py:55-72,
-    src/simulation.cpp:235-282)."""
+    """Uniform current density `amplitude` over a box (a point, line, sheet or
+    volume), sampled by one dipole per cell along each extended axis."""
 
-    def __init__(self, function, center=Vector3(), size=Vector3(), amplitude=1, field_component="Ex"):
+    def __init__(self, function, center=None, size=None, amplitude=1, field_component="Ex"):
         self.function = deepcopy(function)
-        self.center = center.copy()
-        self.size = size.copy()
+        self.center = center.copy() if center is not None else Vector3()
+        self.size = size.copy() if size is not None else Vector3()
         self.amplitude = float(amplitude)
-        self.field_component = field_component
+        self.field_component = _field_name(field_component)
 
     def dump(self, store):
-        pass
+        """A uniform source carries no arrays."""
 
     def get_json(self):
         return {"type": "volume source", "center": self.center.get_json(), "size": self.size.get_json(),
@@ -56,62 +80,39 @@ class Source:
                 "function": self.function.get_json()}
 
 
-class Inhom_Source:
-    """Grid of dipoles with complex amplitudes: |a| scales the pulse and arg(a)
-    delays it by arg(a)/(2 pi f) (ffip/source.py:74-140, src/simulation.cpp:284-336)."""
+class Inhom_Source(NodeGrid):
+    """Dipoles on the `dim` nodes of a box with complex amplitudes a: each
+    dipole's pulse is scaled by |a| and delayed by arg(a) / (2 pi frequency)
+    (src/simulation.cpp:316-333).  The amplitudes are given like a density (an
+    array [z, y, x], a function of the node points, or None for zeros) and go to
+    the datasets "inhom source amplitude <suffix>" and
+    "inhom source phase <suffix>"."""
 
-    def __init__(self, function, amplitude=None, frequency=1, center=Vector3(), size=Vector3(), dim=Vector3(),
+    _grid_array = "amplitude"
+    _grid_dtype = complex
+
+    def __init__(self, function, amplitude=None, frequency=1, center=None, size=None, dim=None,
                  field_component="Ex", suffix="0"):
-        self.dim = dim.round()
+        self._place(center or Vector3(), size or Vector3(), dim or Vector3())
         self.function = deepcopy(function)
         self.frequency = float(frequency)
-        self.center = center.copy()
-        self.size = size.copy()
-        self.field_component = field_component
-        if callable(amplitude):
-            self.amplitude = amplitude(np.stack(np.meshgrid(self.z, self.y, self.x, indexing="ij"), axis=-1))
-        elif isinstance(amplitude, np.ndarray):
-            self.amplitude = amplitude
-        elif amplitude is None:
-            self.amplitude = np.zeros(self.shape, dtype=complex)
-        else:
-            raise ValueError("amplitude input is not numpy array or a function")
+        self.field_component = _field_name(field_component)
+        self.amplitude = self._initial(amplitude)
         self.amplitude_dataset_name = "inhom source amplitude %s" % suffix
         self.phase_dataset_name = "inhom source phase %s" % suffix
-
-    @property
-    def x(self):
-        return np.linspace(self.center.x - self.size.x / 2, self.center.x + self.size.x / 2, int(self.dim.x))
-
-    @property
-    def y(self):
-        return np.linspace(self.center.y - self.size.y / 2, self.center.y + self.size.y / 2, int(self.dim.y))
-
-    @property
-    def z(self):
-        return np.linspace(self.center.z - self.size.z / 2, self.center.z + self.size.z / 2, int(self.dim.z))
 
     @property
     def amplitude(self):
         return self._amp
 
     @amplitude.setter
-    def amplitude(self, val):
-        if not cmp_shape(self.shape, val.shape):
-            raise ValueError("amplitude shape is not compatibale with source dimension")
-        self._amp = np.array(val, copy=True)
-
-    @property
-    def shape(self):
-        return (int(self.dim.z), int(self.dim.y), int(self.dim.x))
-
-    @property
-    def numel(self):
-        return int(self.dim.prod())
+    def amplitude(self, value):
+        self._amp = self._checked(value)
 
     def dump(self, store):
-        store.create_dataset(self.amplitude_dataset_name, data=np.abs(self.amplitude.ravel()))
-        store.create_dataset(self.phase_dataset_name, data=np.angle(self.amplitude.ravel()))
+        a = self.amplitude.ravel()
+        store.create_dataset(self.amplitude_dataset_name, data=np.abs(a))
+        store.create_dataset(self.phase_dataset_name, data=np.angle(a))
 
     def get_json(self):
         return {"type": "inhomogeneous source", "center": self.center.get_json(), "size": self.size.get_json(),

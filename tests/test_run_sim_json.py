@@ -20,7 +20,7 @@ import pytest
 
 import ffip_amd as ffip
 from ffip_amd import _lib
-from ffip_amd.simulation import _DatasetStore
+from ffip_amd.simulation import _Arrays
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 EXE = os.path.join(os.path.dirname(HERE), "ffip_amd", "run_sim_json")
@@ -63,7 +63,7 @@ def _stage(tmp_path, sim, stop):
     """What ffip's Simulation.run leaves before it starts mpirun: config.json and
     the input file, both in the working directory."""
     cfg = sim.get_json(stop_condition=stop)
-    store = _DatasetStore()
+    store = _Arrays()
     sim.dump(store)
     with open(tmp_path / "config.json", "w") as f:
         json.dump(cfg, f, indent=4)

@@ -77,7 +77,7 @@ def _cases():
     # General_Medium_Box through the front end: two Lorentz poles whose amplitudes
     # follow a 1000-entry table over the density
     from ffip_amd.geom import General_Medium_Box, LorentzianSusceptibility, Vector3
-    from ffip_amd.simulation import _DatasetStore
+    from ffip_amd.simulation import _Arrays
     gmb = General_Medium_Box(size=Vector3(8, 6, 4), center=Vector3(0.5, 0, 0), dim=Vector3(9, 7, 5), frequency=0.1,
                              density=np.random.default_rng(11).uniform(0, 1, (5, 7, 9)),
                              epsilon_fun=lambda r: 1 + 2.5 * r + 0.2j * r * (1 - r),
@@ -88,7 +88,7 @@ def _cases():
     cfg["sources"] = [{"type": "volume source", "center": [-6.0, 0.0, 0.0], "size": [0.0, 10.0, 8.0],
                        "amplitude": 1.0, "field component": "Ez", "function": W.gaussian1(0.1)}]
     cfg["fields output"] = [W.dft((6, 0, 0), (0, 10, 8), [0.08, 0.1], "Ez", "a")]
-    store = _DatasetStore()
+    store = _Arrays()
     gmb.dump(store)
     c["general_box"] = (cfg, store.data, [((0.5, 0.5, 0.5), "Ez")])
     return c
