@@ -1727,6 +1727,7 @@ struct EngineImpl {
     void build_tiles();
     int unroll = 2, unroll_mat = 2;  // z planes whose loads are issued together (pole-free / pole kernels)
     int unroll_shell = 0;            // pole kernels on CPML tiles of a split half step (0: unroll_mat)
+    int unroll_plain = 1;            // material-table kernels without poles (per-point or piecewise media)
     void ensure_phasors();
     template <typename T>
     void step_once(hipEvent_t* ev, bool timed_e, bool timed_h);
@@ -1827,11 +1828,16 @@ void EngineImpl::build()
     // design workloads, FP32 and FP64): 8-plane z segments; FP32 kernels issue
     // two planes per load batch (one when a point carries more than two poles,
     // except in the CPML-free kernel), FP64 kernels one (512^3 FP64 E half step
-    // 6151 -> 5630 us against two for the pole kernels)
+    // 6151 -> 5630 us against two for the pole kernels).  Material-table kernels
+    // without poles take one plane per batch in both precisions: at two, the FP32
+    // design-box electric kernel needs 102 VGPRs (4 waves per SIMD) instead of 75
+    // (6), and the 200x200x100 design gradient's electric half step took
+    // 49.2 us per step instead of 44.2.
     lz = 8;
     unroll = fp32 ? 2 : 1;
     unroll_mat = fp32 ? 2 : 1;
     unroll_shell = fp32 ? 0 : 1;
+    unroll_plain = 1;
     // tuning overrides: z segment per workgroup, planes per load batch
     if (const char* e = std::getenv("FFIP_LZ"))
         lz = std::max(1, std::atoi(e));
@@ -1841,6 +1847,8 @@ void EngineImpl::build()
         unroll_mat = std::atoi(e);
     if (const char* e = std::getenv("FFIP_UNROLL_SHELL"))
         unroll_shell = std::atoi(e);
+    if (const char* e = std::getenv("FFIP_UNROLL_PLAIN"))
+        unroll_plain = std::atoi(e);
     // the half-step kernel addresses every per-cell array with a 32-bit byte offset
     if (L.alloc * int64_t(tsz) >= (int64_t(1) << 32))
         throw std::runtime_error("grid too large: one field array must stay below 4 GiB (" +
@@ -2424,7 +2432,7 @@ void EngineImpl::launch_tiles(int h, const HalfArgs<T>& A, const uint32_t* list,
     const bool split = fused || !pml_a[h];
     const int u = pml && split && unroll_shell > 0 && mat && S_slots[h] > 0
                       ? unroll_shell
-                      : (mat && S_slots[h] > 0 ? unroll_mat : unroll);
+                      : (mat ? (S_slots[h] > 0 ? unroll_mat : unroll_plain) : unroll);
     HalfArgs<T> X = A;
     X.tiles = list;
     X.ntiles = count;
