@@ -219,10 +219,13 @@ def _trapezoid(n):
 
 
 class Flux_Region:
-    """Poynting flux Re(E1 conj(H2)) through a plane normal to `direction`
-    (E1, H2 the components next in cyclic order), integrated with the
-    trapezoid rule on a lattice of spacing dx / scale and multiplied by
-    `weight`.  Registers one E and one H monitor on the plane."""
+    """Flux Re(E1 conj(H2)) through a plane normal to `direction` (E1, H2 the
+    components next in cyclic order), integrated with the trapezoid rule on a
+    lattice of spacing dx / scale and multiplied by `weight`.  Registers one E
+    and one H monitor on the plane.  As in the reference
+    (ffip/simulation.py:186-200) this is one term of the normal Poynting
+    component: the other, -Re(E2 conj(H1)), is left out, so the value is the
+    flux of fields polarized with E along E1."""
 
     def __init__(self, sim, center, size, frequency, weight):
         extended = [size[a] != 0 for a in range(3)]
