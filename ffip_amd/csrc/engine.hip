@@ -2393,13 +2393,17 @@ void EngineImpl::build_tiles()
                     const uint32_t t = uint32_t(tx + A.gx * (ty + A.gy * tz));
                     (touch[0][tx] || touch[1][ty] || touch[2][tz] ? sh : in).push_back(t);
                 }
-        // Two launches pay only where the CPML-free kernel gains occupancy (the
-        // material kernels) and each launch still fills the GPU several times
-        // over (measured on MI355X: 256^3 and 512^3 gain 13 %, a 200x200x100
-        // grid of 2275 tiles loses 10 %); otherwise one launch over every tile.
+        // Two launches pay only where the CPML-free kernel gains occupancy and
+        // each launch still fills the GPU several times over (a 200x200x100 grid
+        // of 2275 tiles lost 10 %).  Measured on MI355X (electric half step, one
+        // launch -> two): FP32 with full-grid polarizations (256^3 dispersive,
+        // 74 instead of 109 VGPRs) 240 -> 234 us; FP32 compact (512^3 voxel mix,
+        // 112 and 109 VGPRs: no gain) 2824 -> 2911 us and 640^3 5362 -> 5472 us;
+        // FP64 full grid 414 -> 431 us; FP64 compact 5633 -> 5583 us.  The split
+        // is kept where it gains more than 2 %: FP32 with full-grid polarizations.
         // FFIP_CPML_SPLIT=1 / 0 forces / forbids the split wherever both lists are non-empty
         const char* force = std::getenv("FFIP_CPML_SPLIT");
-        bool split = !s.mat[h].uniform && in.size() + sh.size() >= 4096;
+        bool split = !s.mat[h].uniform && in.size() + sh.size() >= 4096 && fp32 && !pcompact[h];
         if (force)
             split = std::atoi(force) != 0;
         split = split && !sh.empty() && !in.empty();
