@@ -687,10 +687,8 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
 // the same launch.  The fused kernel covers a box of whole tiles clear of every
 // CPML slab; the half-step kernels cover the rest (the CPML shell), the magnetic
 // one before and the electric one after it, so rim values that fall in the shell
-// are read, already new, from the H^{n+1/2} buffer.  Planes of E^n with a
-// one-point rim (three), of H^{n-1/2} and of H^{n+1/2} with the far rim (two
-// each) are staged in LDS, so a single barrier per plane orders them.  The
-// arithmetic and its order are k_half's.
+// are read, already new, from the H^{n+1/2} buffer.  The arithmetic and its order
+// are k_half's.
 template <typename T>
 struct StepArgs {
     HalfArgs<T> e;     // electric half: u = E^n, w = E^{n+1}
@@ -713,42 +711,64 @@ __device__ __forceinline__ T h_value(const StepArgs<T>& S, double d1, double d2,
     return T(double(hold) + S.e.dt * acc * S.h_inv);
 }
 
-// Everything a thread loads for one plane of the fused step: its own E^n and
-// H^{n-1/2}, and the rim points its role asks for (top row j0+BY, right column
-// i0+BX, bottom row j0-1, left column i0-1, two corners).
-template <typename T>
-struct FPlane {
-    T e[3], h[3];
-    T te[3], th[2];  // top (i, j0+BY): Ex, Ey, Ez; Hz, Hx
-    T re[3], rh[2];  // right (i0+BX, j): Ex, Ey, Ez; Hz, Hy
-    T be[2];         // bottom (i, j0-1): Ex, Ez
-    T le[2];         // left (i0-1, j): Ey, Ez
-    T cl, cr;        // Ey (i0-1, j0+BY), Ex (i0+BX, j0-1)
-    uint32_t id;
-    uint32_t sb[3];
+// k_fstep stages the field planes of its tile with direct-to-LDS loads
+// (global_load_lds_dword) issued by a fifth wavefront of the workgroup, D planes
+// ahead of the plane being computed.  The four computing wavefronts hold no
+// field loads in registers, so the bytes a CU keeps in flight are set by the LDS
+// ring (D planes of E^n with a one-point rim and of H^{n-1/2} with the far rim,
+// per workgroup) rather than by the register budget.  The loading wavefront
+// issues nothing but these loads, so waiting for a plane is a static count of
+// outstanding loads (vmcnt), followed by the workgroup barrier that also orders
+// the computing wavefronts' LDS traffic.
+//
+// Ring slot layout, in dwords: for c = 0..2 the E^n region [BY+2][BX+2] (x from
+// i0-1, y from j0-1), then for c = 0..2 the H region [BY+1][BX+1] (x from i0, y
+// from j0); each region is padded to whole loads of 64 dwords, and lane l of the
+// loading wavefront's q-th load of a region writes its dword 64q + l.  An H
+// region holds H^{n-1/2}, except the rims that lie outside the fused box (their
+// H^{n+1/2} was formed by the shell's magnetic kernel) and, for the plane above a
+// segment that ends at the box's top, all of it, which come from H^{n+1/2}.
+template <typename T, int D>
+struct FRing {
+    static constexpr int W = int(sizeof(T) / 4);  // dwords per element
+    static constexpr int EX = BX + 2, EY = BY + 2, HX = BX + 1, HY = BY + 1;
+    static constexpr int ESP = (EX * EY * W + 63) / 64 * 64;  // dwords per E region
+    static constexpr int HSP = (HX * HY * W + 63) / 64 * 64;  // dwords per H region
+    static constexpr int NQE = ESP / 64, NQH = HSP / 64;
+    static constexpr int NQ = 3 * (NQE + NQH);    // loads per plane
+    static constexpr int SLOT = 3 * (ESP + HSP);  // dwords per plane
+    static constexpr int R = D + 2;               // planes in use (2) and in flight (D)
+    static_assert(D >= 1 && NQ * (D - 1) <= 63, "outstanding loads exceed the vmcnt range");
 };
 
-template <typename T, bool MAT, int NPM, bool CP, bool DENSE>
-__global__ void __launch_bounds__(BX * BY) k_step(const StepArgs<T> S)
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+// s_waitcnt vmcnt(n) with the other counters left alone (gfx9 encoding)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
 {
+    static_assert(N >= 0 && N <= 63, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+template <typename T, bool MAT, int NPM, bool CP, bool DENSE, int D>
+__global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
+{
+    using RG = FRing<T, D>;
     const HalfArgs<T>& A = S.e;
     extern __shared__ double lds_tab[];
     MatEntry* smat = reinterpret_cast<MatEntry*>(lds_tab);
     double* spc = lds_tab + 3 * A.nmats;
-    // staged planes: E^n of three planes (x in [i0-1, i0+BX], y in [j0-1, j0+BY]),
-    // H^{n-1/2} and H^{n+1/2} of two (x in [i0, i0+BX], y in [j0, j0+BY])
-    __shared__ T es[3][3][BY + 2][BX + 2];
-    __shared__ T hos[2][3][BY + 1][BX + 1];
-    __shared__ T hs[2][3][BY + 1][BX + 1];
-    const int tx = int(threadIdx.x), ty = int(threadIdx.y);
-    if (MAT) {
-        const int tid = tx + BX * ty;
+    __shared__ uint32_t ring[RG::R * RG::SLOT];
+    const int tid = int(threadIdx.x);
+    const bool loader = tid >= BX * BY;
+    if (MAT && !loader) {
         for (int q = tid; q < A.nmats; q += BX * BY)
             smat[q] = A.mats[q];
         if (NPM > 0)
             for (int q = tid; q < 3 * A.npoles; q += BX * BY)
                 spc[q] = A.coef[q];
-        // visible after the first plane's barrier
+        // visible after the first barrier
     }
     const int b = int(blockIdx.x);
     const int slot = (b & 7) * (A.nblocks >> 3) + (b >> 3);
@@ -760,23 +780,100 @@ __global__ void __launch_bounds__(BX * BY) k_step(const StepArgs<T> S)
     const int fty = tr % S.fn[1];
     const int ftz = tr / S.fn[1];
     const int i0 = S.fo[0] + ftx * BX, j0 = S.fo[1] + fty * BY;
-    const int gtx = (i0 - A.dlo[0]) / BX;  // tile column in the half step's grid
-    const int i = i0 + tx, j = j0 + ty;
     const int kb = S.fo[2] + ftz * A.lz;
     const int ke = min(kb + A.lz - 1, S.fe[2]);
     const int jt = j0 + BY, ir = i0 + BX;
-    const bool top = ty == BY - 1, right = tx == BX - 1, bot = ty == 0, left = tx == 0;
     // rims outside the box: their H^{n+1/2} is in the new buffer already
     const bool top_sh = jt > S.fe[1], right_sh = ir > S.fe[0], above_sh = ke + 1 > S.fe[2];
     // load coordinates stay in [dlo - 1, dhi + 1], inside every array's padding
     auto clx = [&](int x) { return min(max(x, A.dlo[0] - 1), A.dhi[0] + 1); };
     auto cly = [&](int y) { return min(max(y, A.dlo[1] - 1), A.dhi[1] + 1); };
     auto clz = [&](int z) { return min(max(z, A.dlo[2] - 1), A.dhi[2] + 1); };
-    const int il = clx(i), jl = cly(j), itl = clx(ir), ill = clx(i0 - 1), jtl = cly(jt), jbl = cly(j0 - 1);
     const int sz = A.plane;
-    auto off = [&](int x, int y, int z) { return uint32_t(A.origin + x + A.pitch * y + sz * z) * uint32_t(sizeof(T)); };
+    // ring slot of plane p (planes kb-1 .. ke+1)
+    auto slot_base = [&](int p) { return ring + ((p - kb + 1) % RG::R) * RG::SLOT; };
+
+    if (loader) {
+        // ---- the loading wavefront
+        const int lane = tid - BX * BY;
+        uint32_t qe[RG::NQE], qh[RG::NQH];
+        uint32_t hsel = 0;  // H loads whose element is a rim taken from H^{n+1/2}
+#pragma unroll
+        for (int q = 0; q < RG::NQE; ++q) {
+            const int d = q * 64 + lane, e = d / RG::W, dw = d % RG::W;
+            int x = i0, y = j0;
+            if (e < RG::EX * RG::EY) {
+                x = i0 - 1 + e % RG::EX;
+                y = j0 - 1 + e / RG::EX;
+            }
+            qe[q] = (uint32_t(A.origin + clx(x) + A.pitch * cly(y)) * RG::W + uint32_t(dw)) * 4u;
+        }
+#pragma unroll
+        for (int q = 0; q < RG::NQH; ++q) {
+            const int d = q * 64 + lane, e = d / RG::W, dw = d % RG::W;
+            int x = i0, y = j0;
+            if (e < RG::HX * RG::HY) {
+                const int xr = e % RG::HX, yr = e / RG::HX;
+                x = i0 + xr;
+                y = j0 + yr;
+                if ((yr == BY && top_sh) || (xr == BX && right_sh))
+                    hsel |= 1u << q;
+            }
+            qh[q] = (uint32_t(A.origin + clx(x) + A.pitch * cly(y)) * RG::W + uint32_t(dw)) * 4u;
+        }
+        auto issue = [&](int p) {
+            const uint32_t zoff = uint32_t(sz) * uint32_t(clz(p)) * uint32_t(sizeof(T));
+            uint32_t* base = slot_base(p);
+            const bool last_sh = above_sh && p == ke + 1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int q = 0; q < RG::NQE; ++q)
+                    __builtin_amdgcn_global_load_lds(reinterpret_cast<const char*>(A.u[c]) + (qe[q] + zoff),
+                                                     (lds_void_ptr)(base + c * RG::ESP + q * 64), 4, 0, 0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int q = 0; q < RG::NQH; ++q) {
+                    const bool ho = last_sh || ((hsel >> q) & 1u);
+                    const char* src = ho ? reinterpret_cast<const char*>(S.hout[c]) : reinterpret_cast<const char*>(S.hin[c]);
+                    __builtin_amdgcn_global_load_lds(src + (qh[q] + zoff),
+                                                     (lds_void_ptr)(base + 3 * RG::ESP + c * RG::HSP + q * 64), 4, 0, 0);
+                }
+        };
+        // wait until at most `after` planes issued after the one needed are in flight
+        auto wait_planes = [&](int after) {
+            if (after <= 0)
+                wait_vmcnt<0>();
+            else if (D >= 2 && after == 1)
+                wait_vmcnt<(D >= 2 ? RG::NQ : 0)>();
+            else
+                wait_vmcnt<RG::NQ*(D - 1)>();
+        };
+        const int plast = ke + 1;
+        wait_vmcnt<0>();
+        for (int p = kb - 1; p <= min(kb + D - 1, plast); ++p)
+            issue(p);
+        wait_planes(min(D - 1, plast - kb));  // planes kb-1 and kb
+        __syncthreads();
+        if (kb + D <= plast)
+            issue(kb + D);
+        for (int k = kb; k <= ke; ++k) {
+            wait_planes(min(D - 1, ke - k));  // plane k+1
+            __syncthreads();
+            if (k + 1 + D <= plast)
+                issue(k + 1 + D);
+        }
+        return;
+    }
+
+    // ---- the computing wavefronts: thread (tx, ty) of the BX x BY tile
+    const int tx = tid % BX, ty = tid / BX;
+    const int gtx = (i0 - A.dlo[0]) / BX;  // tile column in the half step's grid
+    const int i = i0 + tx, j = j0 + ty;
+    const bool top = ty == BY - 1, right = tx == BX - 1;
+    const int il = clx(i), jl = cly(j);
     const double inv_dx = A.inv_dx;
-    // electric column context (k_half's, no CPML on these tiles)
     ColCtx X;
     X.i = i;
     X.j = j;
@@ -787,7 +884,6 @@ __global__ void __launch_bounds__(BX * BY) k_step(const StepArgs<T> S)
     X.sx = X.sy = -1;
     X.bx = X.cx = X.by = X.cy = 0;
     X.ikx = X.iky = 1;
-    // magnetic points: own (i, j), top rim (i, jt), right rim (ir, j)
     auto hreal_xy = [&](int C, int x, int y) {
         return x >= S.hlo[C][0] && x <= S.hhi[C][0] && y >= S.hlo[C][1] && y <= S.hhi[C][1];
     };
@@ -806,245 +902,146 @@ __global__ void __launch_bounds__(BX * BY) k_step(const StepArgs<T> S)
     // rows past the grid (the last tile row's spare threads) read the last row's entry
     const int seg0 = (min(j, A.dhi[1]) - A.dlo[1]) * A.gx + gtx;
     const int segplane = A.seg_ny * A.gx;
-
-    auto load_plane = [&](int kk, FPlane<T>& P) {
+    auto load_words = [&](int kk, uint32_t& id, uint32_t sb[3]) {
         const int kc = clz(kk);
-        const uint32_t o = off(il, jl, kc);
-#pragma unroll
-        for (int C = 0; C < 3; ++C) {
-            P.e[C] = ldb(A.u[C], o);
-            P.h[C] = ldb(S.hin[C], o);
-        }
         if (MAT) {
             const bool in = mcol && kc >= A.mbox[2][0] && kc <= A.mbox[2][1];
             const uint32_t w = ldb(A.ids, in ? uint32_t(A.origin + il + A.pitch * jl + sz * kc) * 4u : 0u);
-            P.id = in ? w : bg_word;
+            id = in ? w : bg_word;
+        } else {
+            id = 0u;
         }
-        if (POL) {
-            const int kq = min(max(kc, A.dlo[2]), A.dhi[2]);
-#pragma unroll
-            for (int C = 0; C < 3; ++C)
-                P.sb[C] = A.segbase[C][(kq - A.dlo[2]) * segplane + seg0];
-        }
-        if (top) {
-            const uint32_t q = off(il, jtl, kc);
-            if (!top_sh) {
-#pragma unroll
-                for (int C = 0; C < 3; ++C)
-                    P.te[C] = ldb(A.u[C], q);
-            }
-            const T* const* H = top_sh ? S.hout : S.hin;
-            P.th[0] = ldb(H[2], q);
-            P.th[1] = ldb(H[0], q);
-        }
-        if (right) {
-            const uint32_t q = off(itl, jl, kc);
-            if (!right_sh) {
-#pragma unroll
-                for (int C = 0; C < 3; ++C)
-                    P.re[C] = ldb(A.u[C], q);
-            }
-            const T* const* H = right_sh ? S.hout : S.hin;
-            P.rh[0] = ldb(H[2], q);
-            P.rh[1] = ldb(H[1], q);
-        }
-        if (bot) {
-            const uint32_t q = off(il, jbl, kc);
-            P.be[0] = ldb(A.u[0], q);
-            P.be[1] = ldb(A.u[2], q);
-        }
-        if (left) {
-            const uint32_t q = off(ill, jl, kc);
-            P.le[0] = ldb(A.u[1], q);
-            P.le[1] = ldb(A.u[2], q);
-        }
-        if (left && top)
-            P.cl = ldb(A.u[1], off(ill, jtl, kc));
-        if (right && bot)
-            P.cr = ldb(A.u[0], off(itl, jbl, kc));
-    };
-    // stage plane kk: E^n with its rim in es[kk mod 3], H^{n-1/2} with the far rim in hos[kk mod 2]
-    auto write_planes = [&](int kk, const FPlane<T>& P) {
-        T(*E)[BY + 2][BX + 2] = es[(kk + 3) % 3];
-        T(*H)[BY + 1][BX + 1] = hos[kk & 1];
-#pragma unroll
-        for (int C = 0; C < 3; ++C) {
-            E[C][ty + 1][tx + 1] = P.e[C];
-            H[C][ty][tx] = P.h[C];
-        }
-        if (top) {
-            if (!top_sh) {
-#pragma unroll
-                for (int C = 0; C < 3; ++C)
-                    E[C][BY + 1][tx + 1] = P.te[C];
-            }
-            H[2][BY][tx] = P.th[0];
-            H[0][BY][tx] = P.th[1];
-        }
-        if (right) {
-            if (!right_sh) {
-#pragma unroll
-                for (int C = 0; C < 3; ++C)
-                    E[C][ty + 1][BX + 1] = P.re[C];
-            }
-            H[2][ty][BX] = P.rh[0];
-            H[1][ty][BX] = P.rh[1];
-        }
-        if (bot) {
-            E[0][0][tx + 1] = P.be[0];
-            E[2][0][tx + 1] = P.be[1];
-        }
-        if (left) {
-            E[1][ty + 1][0] = P.le[0];
-            E[2][ty + 1][0] = P.le[1];
-        }
-        if (left && top)
-            E[1][BY + 1][0] = P.cl;
-        if (right && bot)
-            E[0][0][BX + 1] = P.cr;
-    };
-    // polarization / dense-medium loads of E^{n+1} plane kk (load_sparse)
-    auto load_sparse_k = [&](int kk, uint32_t id, const uint32_t sb[3], PlaneIn<T, NPM, CP>& L) {
-        ZCtx Z;
-        Z.k = kk;
-        Z.o = X.base + sz * kk;
-        Z.sz = -1;
-        Z.bz = Z.cz = 0;
-        Z.ikz = 1;
-        bool act[3];
+        const int kq = min(max(kc, A.dlo[2]), A.dhi[2]);
 #pragma unroll
         for (int C = 0; C < 3; ++C)
-            act[C] = X.in[C] && kk >= A.lo[C][2] && kk <= A.hi[C][2];
-        load_sparse<T, MAT, NPM, CP, DENSE, false, true>(A, smat, X, Z, id, sb, act, L);
+            sb[C] = POL ? A.segbase[C][(kq - A.dlo[2]) * segplane + seg0] : 0u;
     };
+    // staged values: E^n component c at tile offset (x, y), x in [-1, BX], y in
+    // [-1, BY]; H region component c at (x, y), x in [0, BX], y in [0, BY]
+    constexpr int EC = RG::ESP / RG::W, HC = RG::HSP / RG::W;
+    auto ebase = [&](int p) { return reinterpret_cast<const T*>(slot_base(p)); };
+    auto hbase = [&](int p) { return reinterpret_cast<const T*>(slot_base(p) + 3 * RG::ESP); };
+    auto ev = [&](const T* E, int c, int y, int x) { return double(E[c * EC + (y + 1) * RG::EX + (x + 1)]); };
+    auto hv = [&](const T* G, int c, int y, int x) { return G[c * HC + y * RG::HX + x]; };
+
     // H^{n+1/2} of plane kk (own point and rim points) from the staged E^n planes
-    // kk and kk-1 and H^{n-1/2} of plane kk; fills hs, stores the own point when
+    // kk and kk-1 and the H region of plane kk; fills hs, stores the own point when
     // kk <= ke and returns its values in hn
     auto h_plane = [&](int kk, T hn[3]) {
-        const T(*E)[BY + 2][BX + 2] = es[(kk + 3) % 3];
-        const T(*Ep)[BY + 2][BX + 2] = es[(kk + 2) % 3];
-        const T(*G)[BY + 1][BX + 1] = hos[kk & 1];
-        T(*H)[BY + 1][BX + 1] = hs[kk & 1];
+        const T* E = ebase(kk);
+        const T* Ep = ebase(kk - 1);
+        const T* G = hbase(kk);
+        // H^{n+1/2} replaces H^{n-1/2} in the same region: every element is read
+        // and then written by one thread only
+        T* H = const_cast<T*>(G);
         const bool st = kk <= ke;
-        const double ex = double(E[0][ty + 1][tx + 1]), ey = double(E[1][ty + 1][tx + 1]),
-                     ez = double(E[2][ty + 1][tx + 1]);
+        const double ex = ev(E, 0, ty, tx), ey = ev(E, 1, ty, tx), ez = ev(E, 2, ty, tx);
         const uint32_t o = uint32_t(X.base + sz * kk) * uint32_t(sizeof(T));
         {
             const bool r = hown[0] && hreal_z(0, kk);
-            const double d1 = (double(E[2][ty][tx + 1]) - ez) * inv_dx;
-            const double d2 = (double(Ep[1][ty + 1][tx + 1]) - ey) * inv_dx;
-            const T h0 = G[0][ty][tx];
+            const double d1 = (ev(E, 2, ty - 1, tx) - ez) * inv_dx;
+            const double d2 = (ev(Ep, 1, ty, tx) - ey) * inv_dx;
+            const T h0 = hv(G, 0, ty, tx);
             const T v = r ? h_value(S, d1, d2, h0) : h0;
             hn[0] = v;
-            H[0][ty][tx] = v;
+            H[0 * HC + (ty) * RG::HX + tx] = v;
             if (r && st)
                 stb(S.hout[0], o, v);
         }
         {
             const bool r = hown[1] && hreal_z(1, kk);
-            const double d1 = (double(Ep[0][ty + 1][tx + 1]) - ex) * inv_dx;
-            const double d2 = (double(E[2][ty + 1][tx]) - ez) * inv_dx;
-            const T h0 = G[1][ty][tx];
+            const double d1 = (ev(Ep, 0, ty, tx) - ex) * inv_dx;
+            const double d2 = (ev(E, 2, ty, tx - 1) - ez) * inv_dx;
+            const T h0 = hv(G, 1, ty, tx);
             const T v = r ? h_value(S, d1, d2, h0) : h0;
             hn[1] = v;
-            H[1][ty][tx] = v;
+            H[1 * HC + (ty) * RG::HX + tx] = v;
             if (r && st)
                 stb(S.hout[1], o, v);
         }
         {
             const bool r = hown[2] && hreal_z(2, kk);
-            const double d1 = (double(E[1][ty + 1][tx]) - ey) * inv_dx;
-            const double d2 = (double(E[0][ty][tx + 1]) - ex) * inv_dx;
-            const T h0 = G[2][ty][tx];
+            const double d1 = (ev(E, 1, ty, tx - 1) - ey) * inv_dx;
+            const double d2 = (ev(E, 0, ty - 1, tx) - ex) * inv_dx;
+            const T h0 = hv(G, 2, ty, tx);
             const T v = r ? h_value(S, d1, d2, h0) : h0;
             hn[2] = v;
-            H[2][ty][tx] = v;
+            H[2 * HC + (ty) * RG::HX + tx] = v;
             if (r && st)
                 stb(S.hout[2], o, v);
         }
-        if (top && top_sh) {
-            H[2][BY][tx] = G[2][BY][tx];
-            H[0][BY][tx] = G[0][BY][tx];
-        } else if (top) {
-            const double txv = double(E[0][BY + 1][tx + 1]), tyv = double(E[1][BY + 1][tx + 1]),
-                         tzv = double(E[2][BY + 1][tx + 1]);
+        if (top && !top_sh) {
+            const double txv = ev(E, 0, BY, tx), tyv = ev(E, 1, BY, tx), tzv = ev(E, 2, BY, tx);
             {  // Hz at (i, jt)
-                const double d1 = (double(E[1][BY + 1][tx]) - tyv) * inv_dx;
-                const double d2 = (double(E[0][BY][tx + 1]) - txv) * inv_dx;
-                const T h0 = G[2][BY][tx];
-                H[2][BY][tx] = t_hz && hreal_z(2, kk) ? h_value(S, d1, d2, h0) : h0;
+                const double d1 = (ev(E, 1, BY, tx - 1) - tyv) * inv_dx;
+                const double d2 = (ev(E, 0, BY - 1, tx) - txv) * inv_dx;
+                const T h0 = hv(G, 2, BY, tx);
+                H[2 * HC + (BY) * RG::HX + tx] = t_hz && hreal_z(2, kk) ? h_value(S, d1, d2, h0) : h0;
             }
             {  // Hx at (i, jt)
-                const double d1 = (double(E[2][BY][tx + 1]) - tzv) * inv_dx;
-                const double d2 = (double(Ep[1][BY + 1][tx + 1]) - tyv) * inv_dx;
-                const T h0 = G[0][BY][tx];
-                H[0][BY][tx] = t_hx && hreal_z(0, kk) ? h_value(S, d1, d2, h0) : h0;
+                const double d1 = (ev(E, 2, BY - 1, tx) - tzv) * inv_dx;
+                const double d2 = (ev(Ep, 1, BY, tx) - tyv) * inv_dx;
+                const T h0 = hv(G, 0, BY, tx);
+                H[0 * HC + (BY) * RG::HX + tx] = t_hx && hreal_z(0, kk) ? h_value(S, d1, d2, h0) : h0;
             }
         }
-        if (right && right_sh) {
-            H[2][ty][BX] = G[2][ty][BX];
-            H[1][ty][BX] = G[1][ty][BX];
-        } else if (right) {
-            const double rxv = double(E[0][ty + 1][BX + 1]), ryv = double(E[1][ty + 1][BX + 1]),
-                         rzv = double(E[2][ty + 1][BX + 1]);
+        if (right && !right_sh) {
+            const double rxv = ev(E, 0, ty, BX), ryv = ev(E, 1, ty, BX), rzv = ev(E, 2, ty, BX);
             {  // Hz at (ir, j)
-                const double d1 = (double(E[1][ty + 1][BX]) - ryv) * inv_dx;
-                const double d2 = (double(E[0][ty][BX + 1]) - rxv) * inv_dx;
-                const T h0 = G[2][ty][BX];
-                H[2][ty][BX] = r_hz && hreal_z(2, kk) ? h_value(S, d1, d2, h0) : h0;
+                const double d1 = (ev(E, 1, ty, BX - 1) - ryv) * inv_dx;
+                const double d2 = (ev(E, 0, ty - 1, BX) - rxv) * inv_dx;
+                const T h0 = hv(G, 2, ty, BX);
+                H[2 * HC + (ty) * RG::HX + BX] = r_hz && hreal_z(2, kk) ? h_value(S, d1, d2, h0) : h0;
             }
             {  // Hy at (ir, j)
-                const double d1 = (double(Ep[0][ty + 1][BX + 1]) - rxv) * inv_dx;
-                const double d2 = (double(E[2][ty + 1][BX]) - rzv) * inv_dx;
-                const T h0 = G[1][ty][BX];
-                H[1][ty][BX] = r_hy && hreal_z(1, kk) ? h_value(S, d1, d2, h0) : h0;
+                const double d1 = (ev(Ep, 0, ty, BX) - rxv) * inv_dx;
+                const double d2 = (ev(E, 2, ty, BX - 1) - rzv) * inv_dx;
+                const T h0 = hv(G, 1, ty, BX);
+                H[1 * HC + (ty) * RG::HX + BX] = r_hy && hreal_z(1, kk) ? h_value(S, d1, d2, h0) : h0;
             }
         }
     };
 
-    // ---- prologue: planes kb-1 and kb staged, H^{n+1/2} of plane kb, loads of plane kb+1
-    FPlane<T> PA, PB;
-    load_plane(kb - 1, PA);
-    load_plane(kb, PB);
-    write_planes(kb - 1, PA);
-    write_planes(kb, PB);
-    uint32_t idc = PB.id, sbc[3] = {PB.sb[0], PB.sb[1], PB.sb[2]};
-    __syncthreads();
+    uint32_t idc, sbc[3];
+    load_words(kb, idc, sbc);
+    __syncthreads();  // planes kb-1 and kb staged; material table visible
     {
         T h0[3];
         h_plane(kb, h0);
     }
-    if (!(kb == ke && above_sh))
-        load_plane(kb + 1, PA);
-
-    // One plane: H^{n+1/2} of plane k+1, then E^{n+1} of plane k.  Pc holds the
-    // loads of plane k+1 (issued one plane earlier); the loads of plane k+2 go to
-    // Pn.  The loop alternates the two sets so that no loaded register is copied.
-    auto plane = [&](int k, const FPlane<T>& Pc, FPlane<T>& Pn) {
-        // the plane above a segment that ends at the box's top is the shell's:
-        // its new H is read, not formed
-        const bool last_sh = k == ke && above_sh;
-        if (!last_sh)
-            write_planes(k + 1, Pc);
-        const uint32_t idn = Pc.id, sbn[3] = {Pc.sb[0], Pc.sb[1], Pc.sb[2]};
-        __syncthreads();
-        // polarization loads of plane k first: waiting for them leaves plane k+2's in flight
+    for (int k = kb; k <= ke; ++k) {
+        // this plane's polarization loads and the next plane's material words go
+        // out before the barrier
         PlaneIn<T, NPM, CP> L;
-        load_sparse_k(k, idc, sbc, L);
-        if (k + 1 <= ke && !(k + 1 == ke && above_sh))
-            load_plane(k + 2, Pn);
-        T hn[3];
-        if (!last_sh) {
-            h_plane(k + 1, hn);
-        } else {
-            const uint32_t q = off(il, jl, clz(k + 1));
+        {
+            ZCtx Z;
+            Z.k = k;
+            Z.o = X.base + sz * k;
+            Z.sz = -1;
+            Z.bz = Z.cz = 0;
+            Z.ikz = 1;
+            bool act[3];
 #pragma unroll
             for (int C = 0; C < 3; ++C)
-                hn[C] = ldb(S.hout[C], q);
+                act[C] = X.in[C] && k >= A.lo[C][2] && k <= A.hi[C][2];
+            load_sparse<T, MAT, NPM, CP, DENSE, false, true>(A, smat, X, Z, idc, sbc, act, L);
+        }
+        uint32_t idn, sbn[3];
+        load_words(k + 1, idn, sbn);
+        __syncthreads();  // plane k+1 staged; H^{n+1/2} of plane k complete in hs
+        T hn[3];
+        if (!(k == ke && above_sh)) {
+            h_plane(k + 1, hn);
+        } else {
+            // the plane above is the shell's: its new H was staged into the H region
+            const T* G = hbase(k + 1);
+#pragma unroll
+            for (int C = 0; C < 3; ++C)
+                hn[C] = hv(G, C, ty, tx);
         }
         // E^{n+1} of plane k
-        const T(*E)[BY + 2][BX + 2] = es[(k + 3) % 3];
-        const T(*H)[BY + 1][BX + 1] = hs[k & 1];
+        const T* E = ebase(k);
+        const T* H = hbase(k);
         ZCtx Z;
         Z.k = k;
         Z.o = X.base + sz * k;
@@ -1053,32 +1050,27 @@ __global__ void __launch_bounds__(BX * BY) k_step(const StepArgs<T> S)
         Z.ikz = 1;
 #pragma unroll
         for (int C = 0; C < 3; ++C)
-            L.e[C] = E[C][ty + 1][tx + 1];
-        const double hx = double(H[0][ty][tx]), hy = double(H[1][ty][tx]), hz = double(H[2][ty][tx]);
+            L.e[C] = E[C * EC + (ty + 1) * RG::EX + (tx + 1)];
+        const double hx = double(H[0 * HC + (ty) * RG::HX + tx]), hy = double(H[1 * HC + (ty) * RG::HX + tx]), hz = double(H[2 * HC + (ty) * RG::HX + tx]);
         if (X.in[0] && k >= A.lo[0][2] && k <= A.hi[0][2]) {
-            const double d1 = (double(H[2][ty + 1][tx]) - hz) * inv_dx;
+            const double d1 = (double(H[2 * HC + (ty + 1) * RG::HX + tx]) - hz) * inv_dx;
             const double d2 = (double(hn[1]) - hy) * inv_dx;
             update_comp<T, MAT, NPM, CP, DENSE, 0, false>(A, smat, spc, X, Z, d1, d2, idc, L);
         }
         if (X.in[1] && k >= A.lo[1][2] && k <= A.hi[1][2]) {
             const double d1 = (double(hn[0]) - hx) * inv_dx;
-            const double d2 = (double(H[2][ty][tx + 1]) - hz) * inv_dx;
+            const double d2 = (double(H[2 * HC + (ty) * RG::HX + tx + 1]) - hz) * inv_dx;
             update_comp<T, MAT, NPM, CP, DENSE, 1, false>(A, smat, spc, X, Z, d1, d2, idc, L);
         }
         if (X.in[2] && k >= A.lo[2][2] && k <= A.hi[2][2]) {
-            const double d1 = (double(H[1][ty][tx + 1]) - hy) * inv_dx;
-            const double d2 = (double(H[0][ty + 1][tx]) - hx) * inv_dx;
+            const double d1 = (double(H[1 * HC + (ty) * RG::HX + tx + 1]) - hy) * inv_dx;
+            const double d2 = (double(H[0 * HC + (ty + 1) * RG::HX + tx]) - hx) * inv_dx;
             update_comp<T, MAT, NPM, CP, DENSE, 2, false>(A, smat, spc, X, Z, d1, d2, idc, L);
         }
         idc = idn;
 #pragma unroll
         for (int C = 0; C < 3; ++C)
             sbc[C] = sbn[C];
-    };
-    for (int k = kb; k <= ke; k += 2) {
-        plane(k, PA, PB);
-        if (k + 1 <= ke)
-            plane(k + 1, PB, PA);
     }
 }
 
@@ -1492,6 +1484,9 @@ struct EngineImpl {
     // step's new values and the two swap after every step; the magnetic CPML
     // auxiliaries are double-buffered the same way (hq_b: psi1, psi2 per axis)
     bool fused = false;
+    // planes the fused kernel's loads run ahead of the plane being computed
+    // (FFIP_FDEPTH, 1 or 2)
+    int fdepth = 1;
     void* field_b[6] = {};
     int lzf = 32;
     uint32_t* ftiles = nullptr;  // fused-kernel tiles: 0 .. nft-1 over its box
@@ -1622,6 +1617,30 @@ struct EngineImpl {
     void step_fused(hipEvent_t* ev);
     template <typename T>
     void launch_step(size_t lds, const StepArgs<T>& S0);
+    template <typename T, int D>
+    void launch_fstep(bool mat, int npm, size_t lds, const StepArgs<T>& X)
+    {
+        const dim3 grid(X.e.nblocks, 1, 1), block(BX * BY + 64, 1, 1);
+        if (!mat) {
+            k_fstep<T, false, 0, false, false, D><<<grid, block, 0, stream>>>(X);
+        } else if (dense[0]) {
+            k_fstep<T, true, 0, false, true, D><<<grid, block, lds, stream>>>(X);
+        } else if (npm == 0) {
+            k_fstep<T, true, 0, false, false, D><<<grid, block, lds, stream>>>(X);
+        } else if (pcompact[0]) {
+            switch (npm) {
+            case 1: k_fstep<T, true, 1, true, false, D><<<grid, block, lds, stream>>>(X); break;
+            case 2: k_fstep<T, true, 2, true, false, D><<<grid, block, lds, stream>>>(X); break;
+            default: k_fstep<T, true, 3, true, false, D><<<grid, block, lds, stream>>>(X); break;
+            }
+        } else {
+            switch (npm) {
+            case 1: k_fstep<T, true, 1, false, false, D><<<grid, block, lds, stream>>>(X); break;
+            case 2: k_fstep<T, true, 2, false, false, D><<<grid, block, lds, stream>>>(X); break;
+            default: k_fstep<T, true, 3, false, false, D><<<grid, block, lds, stream>>>(X); break;
+            }
+        }
+    }
     // half-step kernel of half h over a tile list (with or without the CPML paths)
     template <typename T>
     void launch_tiles(int h, const HalfArgs<T>& A, const uint32_t* list, int count, bool pml);
@@ -2191,13 +2210,17 @@ void EngineImpl::build()
     mark("tile lists");
     // fused full-step kernel: only on request (FFIP_FUSED=1) and where the
     // simulation allows it.  Measured on MI355X it is slower than the two
-    // half-step kernels (512^3 voxel mix 4.60 vs 4.36 ms per step, 256^3
-    // dispersive 0.471 vs 0.435 ms): it moves a third fewer field bytes but,
-    // holding a plane of loads in registers (3 waves per SIMD), it keeps fewer
-    // bytes in flight.
+    // half-step kernels: 512^3 voxel mix 4.26 vs 4.09 ms per step (fused
+    // interior 2.76 ms, CPML shell on the half-step kernels 1.50 ms), 256^3
+    // dispersive 0.425 vs 0.379 ms, design gradient 0.735 vs 0.547 s (FP32).  It
+    // moves a third fewer field bytes, but every plane waits for its
+    // polarization loads and a workgroup barrier, and the shell's four small
+    // launches run at under 3 TB/s.
     {
         const char* fe = std::getenv("FFIP_FUSED");
         fused = fe && std::atoi(fe) != 0 && fused_eligible(nullptr);
+        if (const char* d = std::getenv("FFIP_FDEPTH"))
+            fdepth = std::min(2, std::max(1, std::atoi(d)));
     }
     if (fused) {
         for (int c = 0; c < 6; ++c)
@@ -2728,28 +2751,12 @@ void EngineImpl::launch_step(size_t lds, const StepArgs<T>& S0)
     X.e.tiles = ftiles;
     X.e.ntiles = nft;
     X.e.nblocks = ((nft + 7) / 8) * 8;
-    const dim3 grid(X.e.nblocks, 1, 1), block(BX, BY, 1);
     const bool mat = !S->mat[0].uniform;
     const int npm = S_slots[0];
-    if (!mat) {
-        k_step<T, false, 0, false, false><<<grid, block, 0, stream>>>(X);
-    } else if (dense[0]) {
-        k_step<T, true, 0, false, true><<<grid, block, lds, stream>>>(X);
-    } else if (npm == 0) {
-        k_step<T, true, 0, false, false><<<grid, block, lds, stream>>>(X);
-    } else if (pcompact[0]) {
-        switch (npm) {
-        case 1: k_step<T, true, 1, true, false><<<grid, block, lds, stream>>>(X); break;
-        case 2: k_step<T, true, 2, true, false><<<grid, block, lds, stream>>>(X); break;
-        default: k_step<T, true, 3, true, false><<<grid, block, lds, stream>>>(X); break;
-        }
-    } else {
-        switch (npm) {
-        case 1: k_step<T, true, 1, false, false><<<grid, block, lds, stream>>>(X); break;
-        case 2: k_step<T, true, 2, false, false><<<grid, block, lds, stream>>>(X); break;
-        default: k_step<T, true, 3, false, false><<<grid, block, lds, stream>>>(X); break;
-        }
-    }
+    if (fdepth == 1)
+        launch_fstep<T, 1>(mat, npm, lds, X);
+    else
+        launch_fstep<T, 2>(mat, npm, lds, X);
     HIP_OK(hipGetLastError());
 }
 

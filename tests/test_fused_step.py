@@ -24,10 +24,17 @@ pytestmark = pytest.mark.gpu
 COMPS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 
 
-def _run(cfg, ds, steps, fused, monkeypatch, lzf=None, probes=()):
+# the fused kernel's planes are staged by direct-to-LDS loads one or two planes
+# ahead of the plane being computed (FFIP_FDEPTH)
+KERNELS = {"depth1": ("1", "1"), "depth2": ("1", "2")}
+
+
+def _run(cfg, ds, steps, fused, monkeypatch, lzf=None, probes=(), kernel="depth1"):
     """lzf = (planes per half-step tile, planes per fused segment), or None for
     the defaults."""
-    monkeypatch.setenv("FFIP_FUSED", "1" if fused else "0")
+    kind, depth = KERNELS[kernel]
+    monkeypatch.setenv("FFIP_FUSED", kind if fused else "0")
+    monkeypatch.setenv("FFIP_FDEPTH", depth)
     if lzf:
         monkeypatch.setenv("FFIP_LZ", str(lzf[0]))
         monkeypatch.setenv("FFIP_LZF", str(lzf[1]))
@@ -90,23 +97,28 @@ def _cases():
 FUSED_CASES = _cases()
 
 
+@pytest.mark.parametrize("kernel", sorted(KERNELS))
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("name", sorted(FUSED_CASES))
-def test_fused_equals_half_step_kernels(name, prec, monkeypatch):
+def test_fused_equals_half_step_kernels(name, prec, kernel, monkeypatch):
     cfg, ds = FUSED_CASES[name]
     cfg = dict(cfg)
     cfg["precision"] = prec
     probes = [((1.0, 0.5, 0.0), "Ez"), ((-2.5, 1.0, 3.0), "Hy")]
+    print("case %s %s %s" % (name, prec, kernel), flush=True)
     a = _run(cfg, ds, 70, False, monkeypatch, probes=probes)
     for lzf in (None, (3, 6), (2, 2)):
-        b = _run(cfg, ds, 70, True, monkeypatch, lzf=lzf, probes=probes)
-        _compare(a, b, 1e-13 if prec == "fp64" else 1e-6, "%s %s lz=%s" % (name, prec, lzf))
+        b = _run(cfg, ds, 70, True, monkeypatch, lzf=lzf, probes=probes, kernel=kernel)
+        _compare(a, b, 1e-13 if prec == "fp64" else 1e-6, "%s %s %s lz=%s" % (name, prec, kernel, lzf))
 
 
 @pytest.mark.parametrize("name", ["spheres", "voxels", "design"])
-def test_fused_fp64_matches_oracle(name, monkeypatch):
+@pytest.mark.parametrize("kernel", sorted(KERNELS))
+def test_fused_fp64_matches_oracle(name, kernel, monkeypatch):
     cfg, ds = FUSED_CASES[name]
-    monkeypatch.setenv("FFIP_FUSED", "1")
+    kind, depth = KERNELS[kernel]
+    monkeypatch.setenv("FFIP_FUSED", kind)
+    monkeypatch.setenv("FFIP_FDEPTH", depth)
     monkeypatch.setenv("FFIP_LZ", "3")
     monkeypatch.setenv("FFIP_LZF", "6")
     sim = LibSim(cfg, ds)
