@@ -5,6 +5,12 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# the oracle's C loop runs on OpenMP threads; several pytest-xdist workers each
+# starting a team as wide as the machine oversubscribe it (a 1 s test then takes
+# minutes), so each worker gets an equal share of the cores
+if os.environ.get("PYTEST_XDIST_WORKER") and "OMP_NUM_THREADS" not in os.environ:
+    _workers = int(os.environ.get("PYTEST_XDIST_WORKER_COUNT", "1") or 1)
+    os.environ["OMP_NUM_THREADS"] = str(max(1, (os.cpu_count() or 1) // max(1, _workers)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 

@@ -104,6 +104,56 @@ def test_cpml_split_kernels_match_oracle(name, monkeypatch):
     _compare(cfg, ds, probes, steps=150, label=name + "-split")
 
 
+def _widened(cfg, nx):
+    """The same configuration on a grid nx cells wide."""
+    cfg = dict(cfg)
+    cfg["size"] = [float(nx)] + [float(v) for v in cfg["size"][1:]]
+    return cfg
+
+
+def _magnetic(nx=20):
+    cfg = W.base((nx, 18, 16), 4)
+    m = W.medium(2.5, e_sus=[W.lorentz(0.09, 0.02, 0.8)], e_cond=0.02)
+    m["mu"] = 1.5
+    m["magnetic conductivity"] = 0.03
+    m["magnetic susceptibility"] = [W.lorentz(0.11, 0.03, 0.4)]
+    cfg["geometry"] = [{"type": "box", "center": [1, 0, 0], "size": [8, 6, 6], "medium": m},
+                       {"type": "sphere", "center": [-4, 2, 1], "radius": 2.5, "medium": W.medium(4.0)}]
+    cfg["sources"] = [{"type": "volume source", "center": [-2.0, 0.5, 0.0], "size": [0, 0, 0], "amplitude": 1.0,
+                       "field component": "Ey", "function": W.gaussian1(0.1)},
+                      {"type": "volume source", "center": [3.0, 0.0, 1.5], "size": [0, 2, 0], "amplitude": 0.5,
+                       "field component": "Hz", "function": W.gaussian1(0.08)}]
+    return cfg
+
+
+def _tail_cases():
+    c = {"spheres64": (W.dispersive_spheres(n=64, nspheres=6, pml_cells=6, nfreq=3, precision="fp64"), None, [])}
+    for name in ("design", "general_box", "odd_grid", "voxels"):
+        cfg, ds, probes = CASES[name]
+        c[name + "32"] = (_widened(cfg, 32) if name != "voxels" else cfg, ds, probes)
+    cfg = _magnetic(32)
+    f = [0.07, 0.1]
+    # flux-density and field monitors across the whole x range, boundary faces included
+    cfg["fields output"] = [W.dft((0, 0, 0), (32, 4, 0), f, comp, "t%d" % q)
+                            for q, comp in enumerate(["Ey", "Dy", "Bz", "Hz", "Dx", "Hx"])]
+    c["magnetic32"] = (cfg, None, [((15.5, 0.5, 0.0), "Ey"), ((16.0, 1.0, 0.5), "Hz")])
+    return c
+
+
+TAIL_CASES = _tail_cases()
+
+
+@pytest.mark.parametrize("name", sorted(TAIL_CASES))
+def test_power_of_two_grids_match_oracle(name):
+    """Grids of 2^k cells along x have 2^k + 1 lattice points there, so the last
+    32-point tile column holds one point per row (the layout of the benchmark
+    grids): FP64 against the oracle, with poles in both polarization layouts,
+    per-point media, magnetic media and flux-density monitors reaching the
+    boundary faces."""
+    cfg, ds, probes = TAIL_CASES[name]
+    _compare(cfg, ds, probes, steps=150, label=name)
+
+
 @pytest.mark.parametrize("split", ["0", "1"])
 def test_dispersive_default_medium(split, monkeypatch):
     """A dispersive default medium with pole-free boxes in it: points outside the
@@ -133,17 +183,7 @@ def test_dispersive_spheres_fp64_and_fp32():
 def test_magnetic_media_and_flux_density_monitors():
     """mu != 1, magnetic conductivity and a magnetic Lorentz pole run the ADE on
     the H half step; D and B monitors sample eps_inf*E + sum P (B likewise)."""
-    cfg = W.base((20, 18, 16), 4)
-    m = W.medium(2.5, e_sus=[W.lorentz(0.09, 0.02, 0.8)], e_cond=0.02)
-    m["mu"] = 1.5
-    m["magnetic conductivity"] = 0.03
-    m["magnetic susceptibility"] = [W.lorentz(0.11, 0.03, 0.4)]
-    cfg["geometry"] = [{"type": "box", "center": [1, 0, 0], "size": [8, 6, 6], "medium": m},
-                       {"type": "sphere", "center": [-4, 2, 1], "radius": 2.5, "medium": W.medium(4.0)}]
-    cfg["sources"] = [{"type": "volume source", "center": [-2.0, 0.5, 0.0], "size": [0, 0, 0], "amplitude": 1.0,
-                       "field component": "Ey", "function": W.gaussian1(0.1)},
-                      {"type": "volume source", "center": [3.0, 0.0, 1.5], "size": [0, 2, 0], "amplitude": 0.5,
-                       "field component": "Hz", "function": W.gaussian1(0.08)}]
+    cfg = _magnetic()
     f = [0.07, 0.1]
     cfg["fields output"] = [W.dft((1, 0, 0), (6, 4, 0), f, c, "g%d" % q)
                             for q, c in enumerate(["Ey", "Dy", "Bz", "Hz", "Dx", "Hx"])]
