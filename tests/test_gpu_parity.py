@@ -235,6 +235,34 @@ def test_dft_sample_batches_are_flushed(nf):
                     assert e <= tol, (prec, nf, stage, k, part, e)
 
 
+def test_monitors_at_sources_volumes_and_past_the_grid():
+    """E/H monitors read after 77 steps (13 samples pending in the ring): a plane
+    through a current sheet and one through a dipole (the samples are the values
+    after the point sources), an H plane, a volume and a plane reaching past the
+    grid, against the oracle in both storage types."""
+    freqs = [0.07, 0.1, 0.13]
+    cfg = W.vacuum_dipole(n=32, steps=90, pml_cells=6, freqs=freqs, plane_z=5)
+    cfg["sources"].append({"type": "volume source", "center": [0.0, 0.0, -3.0], "size": [10.0, 8.0, 0.0],
+                           "amplitude": 0.7, "field component": "Ex", "function": W.gaussian1(0.08)})
+    cfg["fields output"] += [W.dft((0, 0, -3), (12, 10, 0), freqs, "Ex", "sheet"),
+                             W.dft((0, 0, 0), (10, 10, 0), freqs, "Ez", "dipole plane"),
+                             W.dft((0, 0, 4), (16, 16, 0), freqs, "Hy", "h plane"),
+                             W.dft((2, 1, 0), (6, 5, 4), freqs, "Ey", "volume"),
+                             W.dft((0, 0, 2), (40, 40, 0), freqs, "Ez", "past the grid")]
+    ora = OracleSim(cfg)
+    ora.run_steps(77)
+    for prec, tol in (("fp64", TOL64), ("fp32", 1e-5)):
+        sim = LibSim(dict(cfg, precision=prec))
+        sim.step(77)
+        for k in range(sim.dft_count()):
+            a, b = sim.dft(k), ora.dft_output(k)
+            assert np.abs(b["real"]).max() > 0
+            for part in ("real", "imag"):
+                e = rel_l2(a[part], b[part])
+                assert e <= tol, (prec, k, part, e)
+        sim.close()
+
+
 def test_continuation_equals_single_run():
     cfg = W.vacuum_dipole(n=20, steps=100, pml_cells=4, plane_z=3)
     a = LibSim(cfg)
