@@ -725,9 +725,12 @@ __device__ __forceinline__ T h_value(const StepArgs<T>& S, double d1, double d2,
 // i0-1, y from j0-1), then for c = 0..2 the H region [BY+1][BX+1] (x from i0, y
 // from j0); each region is padded to whole loads of 64 dwords, and lane l of the
 // loading wavefront's q-th load of a region writes its dword 64q + l.  An H
-// region holds H^{n-1/2}, except the rims that lie outside the fused box (their
-// H^{n+1/2} was formed by the shell's magnetic kernel) and, for the plane above a
-// segment that ends at the box's top, all of it, which come from H^{n+1/2}.
+// region is loaded with H^{n-1/2}, except the rims that lie outside the fused box
+// (their H^{n+1/2} was formed by the shell's magnetic kernel) and, for the plane
+// above a segment that ends at the box's top, all of it, which come from
+// H^{n+1/2}; the computing wavefronts then write H^{n+1/2} over it in place (each
+// element is read and written by one thread), and the next plane's electric
+// update reads its neighbours there after the plane's barrier.
 template <typename T, int D>
 struct FRing {
     static constexpr int W = int(sizeof(T) / 4);  // dwords per element
@@ -836,7 +839,8 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
 #pragma unroll
                 for (int q = 0; q < RG::NQH; ++q) {
                     const bool ho = last_sh || ((hsel >> q) & 1u);
-                    const char* src = ho ? reinterpret_cast<const char*>(S.hout[c]) : reinterpret_cast<const char*>(S.hin[c]);
+                    const char* src = ho ? reinterpret_cast<const char*>(S.hout[c])
+                                         : reinterpret_cast<const char*>(S.hin[c]);
                     __builtin_amdgcn_global_load_lds(src + (qh[q] + zoff),
                                                      (lds_void_ptr)(base + 3 * RG::ESP + c * RG::HSP + q * 64), 4, 0, 0);
                 }
@@ -922,7 +926,8 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
     auto ebase = [&](int p) { return reinterpret_cast<const T*>(slot_base(p)); };
     auto hbase = [&](int p) { return reinterpret_cast<const T*>(slot_base(p) + 3 * RG::ESP); };
     auto ev = [&](const T* E, int c, int y, int x) { return double(E[c * EC + (y + 1) * RG::EX + (x + 1)]); };
-    auto hv = [&](const T* G, int c, int y, int x) { return G[c * HC + y * RG::HX + x]; };
+    auto hix = [&](int c, int y, int x) { return c * HC + y * RG::HX + x; };
+    auto hv = [&](const T* G, int c, int y, int x) { return G[hix(c, y, x)]; };
 
     // H^{n+1/2} of plane kk (own point and rim points) from the staged E^n planes
     // kk and kk-1 and the H region of plane kk; fills hs, stores the own point when
@@ -944,7 +949,7 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
             const T h0 = hv(G, 0, ty, tx);
             const T v = r ? h_value(S, d1, d2, h0) : h0;
             hn[0] = v;
-            H[0 * HC + (ty) * RG::HX + tx] = v;
+            H[hix(0, ty, tx)] = v;
             if (r && st)
                 stb(S.hout[0], o, v);
         }
@@ -955,7 +960,7 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
             const T h0 = hv(G, 1, ty, tx);
             const T v = r ? h_value(S, d1, d2, h0) : h0;
             hn[1] = v;
-            H[1 * HC + (ty) * RG::HX + tx] = v;
+            H[hix(1, ty, tx)] = v;
             if (r && st)
                 stb(S.hout[1], o, v);
         }
@@ -966,7 +971,7 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
             const T h0 = hv(G, 2, ty, tx);
             const T v = r ? h_value(S, d1, d2, h0) : h0;
             hn[2] = v;
-            H[2 * HC + (ty) * RG::HX + tx] = v;
+            H[hix(2, ty, tx)] = v;
             if (r && st)
                 stb(S.hout[2], o, v);
         }
@@ -976,13 +981,13 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
                 const double d1 = (ev(E, 1, BY, tx - 1) - tyv) * inv_dx;
                 const double d2 = (ev(E, 0, BY - 1, tx) - txv) * inv_dx;
                 const T h0 = hv(G, 2, BY, tx);
-                H[2 * HC + (BY) * RG::HX + tx] = t_hz && hreal_z(2, kk) ? h_value(S, d1, d2, h0) : h0;
+                H[hix(2, BY, tx)] = t_hz && hreal_z(2, kk) ? h_value(S, d1, d2, h0) : h0;
             }
             {  // Hx at (i, jt)
                 const double d1 = (ev(E, 2, BY - 1, tx) - tzv) * inv_dx;
                 const double d2 = (ev(Ep, 1, BY, tx) - tyv) * inv_dx;
                 const T h0 = hv(G, 0, BY, tx);
-                H[0 * HC + (BY) * RG::HX + tx] = t_hx && hreal_z(0, kk) ? h_value(S, d1, d2, h0) : h0;
+                H[hix(0, BY, tx)] = t_hx && hreal_z(0, kk) ? h_value(S, d1, d2, h0) : h0;
             }
         }
         if (right && !right_sh) {
@@ -991,13 +996,13 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
                 const double d1 = (ev(E, 1, ty, BX - 1) - ryv) * inv_dx;
                 const double d2 = (ev(E, 0, ty - 1, BX) - rxv) * inv_dx;
                 const T h0 = hv(G, 2, ty, BX);
-                H[2 * HC + (ty) * RG::HX + BX] = r_hz && hreal_z(2, kk) ? h_value(S, d1, d2, h0) : h0;
+                H[hix(2, ty, BX)] = r_hz && hreal_z(2, kk) ? h_value(S, d1, d2, h0) : h0;
             }
             {  // Hy at (ir, j)
                 const double d1 = (ev(Ep, 0, ty, BX) - rxv) * inv_dx;
                 const double d2 = (ev(E, 2, ty, BX - 1) - rzv) * inv_dx;
                 const T h0 = hv(G, 1, ty, BX);
-                H[1 * HC + (ty) * RG::HX + BX] = r_hy && hreal_z(1, kk) ? h_value(S, d1, d2, h0) : h0;
+                H[hix(1, ty, BX)] = r_hy && hreal_z(1, kk) ? h_value(S, d1, d2, h0) : h0;
             }
         }
     };
@@ -1051,20 +1056,21 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
 #pragma unroll
         for (int C = 0; C < 3; ++C)
             L.e[C] = E[C * EC + (ty + 1) * RG::EX + (tx + 1)];
-        const double hx = double(H[0 * HC + (ty) * RG::HX + tx]), hy = double(H[1 * HC + (ty) * RG::HX + tx]), hz = double(H[2 * HC + (ty) * RG::HX + tx]);
+        const double hx = double(hv(H, 0, ty, tx)), hy = double(hv(H, 1, ty, tx));
+        const double hz = double(hv(H, 2, ty, tx));
         if (X.in[0] && k >= A.lo[0][2] && k <= A.hi[0][2]) {
-            const double d1 = (double(H[2 * HC + (ty + 1) * RG::HX + tx]) - hz) * inv_dx;
+            const double d1 = (double(hv(H, 2, ty + 1, tx)) - hz) * inv_dx;
             const double d2 = (double(hn[1]) - hy) * inv_dx;
             update_comp<T, MAT, NPM, CP, DENSE, 0, false>(A, smat, spc, X, Z, d1, d2, idc, L);
         }
         if (X.in[1] && k >= A.lo[1][2] && k <= A.hi[1][2]) {
             const double d1 = (double(hn[0]) - hx) * inv_dx;
-            const double d2 = (double(H[2 * HC + (ty) * RG::HX + tx + 1]) - hz) * inv_dx;
+            const double d2 = (double(hv(H, 2, ty, tx + 1)) - hz) * inv_dx;
             update_comp<T, MAT, NPM, CP, DENSE, 1, false>(A, smat, spc, X, Z, d1, d2, idc, L);
         }
         if (X.in[2] && k >= A.lo[2][2] && k <= A.hi[2][2]) {
-            const double d1 = (double(H[1 * HC + (ty) * RG::HX + tx + 1]) - hy) * inv_dx;
-            const double d2 = (double(H[0 * HC + (ty + 1) * RG::HX + tx]) - hx) * inv_dx;
+            const double d1 = (double(hv(H, 1, ty, tx + 1)) - hy) * inv_dx;
+            const double d2 = (double(hv(H, 0, ty + 1, tx)) - hx) * inv_dx;
             update_comp<T, MAT, NPM, CP, DENSE, 2, false>(A, smat, spc, X, Z, d1, d2, idc, L);
         }
         idc = idn;
