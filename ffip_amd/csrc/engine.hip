@@ -134,7 +134,8 @@ struct HalfArgs {
     //             pcomp[c], the row segment (k, j, x tile) holds its points' poles
     //             from segbase[c][((k - dlo_z) * seg_ny + (j - dlo_y)) * gx + tx],
     //             pole-major: pole 0 of the segment's points (in x order) that
-    //             have it, then pole 1, and so on.
+    //             have it, then pole 1, and so on.  Segments are stored tile by
+    //             tile (plan_polarization), which the kernels do not depend on.
     T* pcur;
     T* pprev;
     int64_t pslice;
@@ -1777,7 +1778,7 @@ EngineImpl::~EngineImpl()
         (void)hipStreamDestroy(stream);
 }
 
-PolLayoutPlan plan_polarization(const Setup& s, int h)
+PolLayoutPlan plan_polarization(const Setup& s, int h, int lz)
 {
     const Layout& L = s.L;
     const HalfMaterial& M = s.mat[h];
@@ -1798,32 +1799,38 @@ PolLayoutPlan plan_polarization(const Setup& s, int h)
     P.ny = std::max(0, P.dhi[1] - P.dlo[1] + 1);
     P.nz = std::max(0, P.dhi[2] - P.dlo[2] + 1);
     const size_t nseg = size_t(P.gx) * P.ny * P.nz;
+    lz = std::max(1, lz);
     for (int c = 0; c < 3; ++c) {
         const int comp = h * 3 + c;
         std::vector<uint32_t>& sb = P.segbase[c];
         sb.assign(std::max<size_t>(nseg, 1), 0u);
         int64_t run = 0;
-        size_t seg = 0;
-        for (int k = P.dlo[2]; k <= P.dhi[2]; ++k)
-            for (int j = P.dlo[1]; j <= P.dhi[1]; ++j)
-                for (int tx = 0; tx < P.gx; ++tx, ++seg) {
-                    sb[seg] = uint32_t(run);
-                    if (j < L.lo[comp][1] || j > L.hi[comp][1] || k < L.lo[comp][2] || k > L.hi[comp][2])
-                        continue;
-                    const int i0 = std::max(P.dlo[0] + tx * BX, L.lo[comp][0]);
-                    const int i1 = std::min(P.dlo[0] + tx * BX + BX - 1, L.hi[comp][0]);
-                    int most = 0, n = 0;
-                    for (int i = i0; i <= i1; ++i) {
-                        const int id = M.ids[c][L.index(i, j, k)];
-                        const int np = id == 255 ? M.dense[c].npoles : M.table.npoles[id];
-                        n += np;
-                        most = std::max(most, np);
-                    }
-                    run += n;
-                    P.full_bytes += double(most) * BX * double(tsz);
-                    // a segment's entries are contiguous: whole lines but a partial one
-                    P.compact_bytes += n ? double(n) * double(tsz) + 64.0 : 0.0;
-                }
+        // entries in tile order (z segment, y tile, x tile), then plane and row
+        // within the tile: the segments one workgroup reads for one plane are
+        // contiguous, and so are its successive planes
+        for (int z0 = P.dlo[2]; z0 <= P.dhi[2]; z0 += lz)
+            for (int y0 = P.dlo[1]; y0 <= P.dhi[1]; y0 += BY)
+                for (int tx = 0; tx < P.gx; ++tx)
+                    for (int k = z0; k <= std::min(z0 + lz - 1, P.dhi[2]); ++k)
+                        for (int j = y0; j <= std::min(y0 + BY - 1, P.dhi[1]); ++j) {
+                            const size_t seg = (size_t(k - P.dlo[2]) * P.ny + size_t(j - P.dlo[1])) * P.gx + tx;
+                            sb[seg] = uint32_t(run);
+                            if (j < L.lo[comp][1] || j > L.hi[comp][1] || k < L.lo[comp][2] || k > L.hi[comp][2])
+                                continue;
+                            const int i0 = std::max(P.dlo[0] + tx * BX, L.lo[comp][0]);
+                            const int i1 = std::min(P.dlo[0] + tx * BX + BX - 1, L.hi[comp][0]);
+                            int most = 0, n = 0;
+                            for (int i = i0; i <= i1; ++i) {
+                                const int id = M.ids[c][L.index(i, j, k)];
+                                const int np = id == 255 ? M.dense[c].npoles : M.table.npoles[id];
+                                n += np;
+                                most = std::max(most, np);
+                            }
+                            run += n;
+                            P.full_bytes += double(most) * BX * double(tsz);
+                            // a segment's entries are contiguous: whole lines but a partial one
+                            P.compact_bytes += n ? double(n) * double(tsz) + 64.0 : 0.0;
+                        }
         if (run * int64_t(tsz) >= (int64_t(1) << 32))
             throw std::runtime_error("polarization state of one component exceeds 4 GiB");
         P.comp_start[c] = P.total;
@@ -1892,7 +1899,7 @@ void EngineImpl::build()
         if (S_slots[h] > MAXS)
             throw std::runtime_error("more than 8 polarization poles at one point");
         if (S_slots[h] > 0) {
-            PolLayoutPlan P = plan_polarization(s, h);
+            PolLayoutPlan P = plan_polarization(s, h, lz);
             pcompact[h] = P.compact;
             size_t cnt;
             if (P.compact) {
@@ -2428,8 +2435,11 @@ void EngineImpl::build_tiles()
         // launch -> two): FP32 with full-grid polarizations (256^3 dispersive,
         // 74 instead of 109 VGPRs) 240 -> 234 us; FP32 compact (512^3 voxel mix,
         // 112 and 109 VGPRs: no gain) 2824 -> 2911 us and 640^3 5362 -> 5472 us;
-        // FP64 full grid 414 -> 431 us; FP64 compact 5633 -> 5583 us.  The split
-        // is kept where it gains more than 2 %: FP32 with full-grid polarizations.
+        // FP64 full grid 414 -> 431 us; FP64 compact 5633 -> 5583 us.  Measured
+        // again after the compact kernels went to one plane per batch (75 VGPRs
+        // without the CPML paths, 109 with them): 512^3 2747 -> 2757 us, 640^3
+        // 5230 -> 5206 us.  The split is kept where it gains more than 2 %: FP32
+        // with full-grid polarizations.
         // FFIP_CPML_SPLIT=1 / 0 forces / forbids the split wherever both lists are non-empty
         const char* force = std::getenv("FFIP_CPML_SPLIT");
         bool split = !s.mat[h].uniform && in.size() + sh.size() >= 4096 && fp32 && !pcompact[h];

@@ -28,7 +28,7 @@ struct PolLayoutPlan {
     int64_t total = 0;                        // compact: entries of the half
     std::vector<uint32_t> segbase[3];         // compact: first entry of every segment
 };
-PolLayoutPlan plan_polarization(const Setup& S, int half);
+PolLayoutPlan plan_polarization(const Setup& S, int half, int lz = 8);
 
 struct KernelTiming {
     double total_ms = 0;  // wall time of the timed region (HIP events)
