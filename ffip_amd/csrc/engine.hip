@@ -221,7 +221,9 @@ __device__ __forceinline__ void row_rank(bool pred, uint32_t& rank, uint32_t& co
 }
 
 // CPML auxiliary offsets (slab-only arrays, src/fields.cpp:89-109): the slab along
-// axis a stores [W][n_b][n_c] with the slab position replacing that axis.
+// axis a holds the points of its W slab positions, the slab position replacing
+// that axis.  The y and z slabs keep x-rows of the field arrays' padded pitch, so
+// a tile's row starts on a 128-byte line as it does in the field arrays.
 template <typename T>
 __device__ __forceinline__ int psi_off_x(const HalfArgs<T>& A, const ColCtx& X, int k)
 {
@@ -230,12 +232,12 @@ __device__ __forceinline__ int psi_off_x(const HalfArgs<T>& A, const ColCtx& X, 
 template <typename T>
 __device__ __forceinline__ int psi_off_y(const HalfArgs<T>& A, const ColCtx& X, int k)
 {
-    return X.i + A.n[0] * (X.sy + A.pml[1].W * k);
+    return X.i + A.pitch * (X.sy + A.pml[1].W * k);
 }
 template <typename T>
 __device__ __forceinline__ int psi_off_z(const HalfArgs<T>& A, const ColCtx& X, int sz)
 {
-    return X.i + A.n[0] * (X.j + A.n[1] * sz);
+    return X.i + A.pitch * (X.j + A.n[1] * sz);
 }
 
 template <typename T>
@@ -1488,8 +1490,8 @@ struct EngineImpl {
 
     void* field[6] = {};
     // fused full-step mode: field holds E^n / H^{n-1/2}, field_b receives the
-    // step's new values and the two swap after every step; the magnetic CPML
-    // auxiliaries are double-buffered the same way (hq_b: psi1, psi2 per axis)
+    // step's new values and the two swap after every step (the CPML auxiliaries
+    // are read and written only by the shell's half-step kernels, in place)
     bool fused = false;
     // planes the fused kernel's loads run ahead of the plane being computed
     // (FFIP_FDEPTH, 1 or 2)
@@ -1971,10 +1973,11 @@ void EngineImpl::build()
             D.b = upload(P.b[par]);
             D.c = upload(P.c[par]);
             D.ik = upload(P.ik[par]);
+            // [n_z][W][pitch] (y slab), [W][n_y][pitch] (z slab), [n_z][n_y][W] (x slab)
             size_t cnt = size_t(D.W);
             for (int b = 0; b < 3; ++b)
                 if (b != a)
-                    cnt *= size_t(L.n[b]);
+                    cnt *= b == 0 ? size_t(L.pitch) : size_t(L.n[b]);
             D.psi1 = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
             D.psi2 = fp32 ? (void*)dalloc<float>(cnt) : (void*)dalloc<double>(cnt);
         }
