@@ -104,6 +104,7 @@ struct HalfArgs {
     T* w[3];        // where their new values go (= u, or the other buffer of a fused step)
     const T* s[3];
     int pitch, plane;
+    int tail0;  // first packed x index (Layout::tail0)
     int n[3];
     int lo[3][3], hi[3][3];
     int dlo[3], dhi[3];
@@ -164,6 +165,7 @@ __device__ __forceinline__ double gaussian1(double t, double width)
 // coefficients along x and y, and which of the three components live here.
 struct ColCtx {
     int i, j, base;
+    int dx, dy;  // element offsets of the x and y neighbours the half reads (E: +1, H: -1)
     int sx, sy;  // slab positions (-1 outside the CPML)
     double bx, cx, ikx, by, cy, iky;
     bool in[3];
@@ -187,6 +189,17 @@ template <typename T>
 __device__ __forceinline__ void stb(T* base, uint32_t off, T v)
 {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
+// Element of lattice index (i, j, k) relative to index (0, 0, 0) in every per-cell
+// array (Layout::index less its origin): rows of `pitch`, planes of `plane`, and
+// from x index tail0 on the packed last line shared by TAIL_GROUP rows.
+__device__ __forceinline__ int64_t lat_off(int64_t pitch, int64_t plane, int tail0, int i, int j, int k)
+{
+    if (i < tail0)
+        return i + pitch * j + plane * k;
+    const int m = (j + 1) % TAIL_GROUP;
+    return tail0 + 2 * m + (i - tail0) + pitch * (j - m) + plane * k;
 }
 
 // Everything one thread reads for one plane.  All of it is requested before any
@@ -250,26 +263,24 @@ __device__ __forceinline__ int64_t dense_off(const HalfArgs<T>& A, int C, int i,
 
 // Stencil operands and updated components of one plane (loads every thread
 // issues; o is clamped into the segment so they are always in bounds).
+// dx, dy: the column's x and y neighbour offsets (ColCtx)
 template <typename T, bool IS_E, int NPM, bool CP>
-__device__ __forceinline__ void load_fields(const HalfArgs<T>& A, int o, PlaneIn<T, NPM, CP>& L)
+__device__ __forceinline__ void load_fields(const HalfArgs<T>& A, int o, int dx, int dy, PlaneIn<T, NPM, CP>& L)
 {
     constexpr uint32_t E = sizeof(T);
-    const uint32_t b = uint32_t(o) * E, by = uint32_t(A.pitch) * E, bz = uint32_t(A.plane) * E;
+    const uint32_t b = uint32_t(o) * E, bz = uint32_t(A.plane) * E;
+    const uint32_t bx = b + uint32_t(dx) * E, by = b + uint32_t(dy) * E;
     if (IS_E) {
         L.n0 = ldb(A.s[0], b + bz);
         L.n1 = ldb(A.s[1], b + bz);
-        L.v2a = ldb(A.s[2], b + E);
-        L.v2b = ldb(A.s[2], b + by);
-        L.v1a = ldb(A.s[1], b + E);
-        L.v0b = ldb(A.s[0], b + by);
     } else {
         L.n0 = ldb(A.s[0], b);
         L.n1 = ldb(A.s[1], b);
-        L.v2a = ldb(A.s[2], b - E);
-        L.v2b = ldb(A.s[2], b - by);
-        L.v1a = ldb(A.s[1], b - E);
-        L.v0b = ldb(A.s[0], b - by);
     }
+    L.v2a = ldb(A.s[2], bx);
+    L.v2b = ldb(A.s[2], by);
+    L.v1a = ldb(A.s[1], bx);
+    L.v0b = ldb(A.s[0], by);
     L.v2 = ldb(A.s[2], b);
 #pragma unroll
     for (int C = 0; C < 3; ++C)
@@ -533,7 +544,12 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
     ColCtx X;
     X.i = i;
     X.j = j;
-    X.base = A.origin + i + A.pitch * j;
+    X.base = A.origin + int(lat_off(A.pitch, A.plane, A.tail0, i, j, 0));
+    {
+        constexpr int u = IS_E ? 1 : -1;
+        X.dx = A.origin + int(lat_off(A.pitch, A.plane, A.tail0, i + u, j, 0)) - X.base;
+        X.dy = A.origin + int(lat_off(A.pitch, A.plane, A.tail0, i, j + u, 0)) - X.base;
+    }
 #pragma unroll
     for (int C = 0; C < 3; ++C)
         X.in[C] = i >= A.lo[C][0] && i <= A.hi[C][0] && j >= A.lo[C][1] && j <= A.hi[C][1];
@@ -642,7 +658,7 @@ __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
         }
 #pragma unroll
         for (int q = 0; q < U; ++q)
-            load_fields<T, IS_E, NPM, CP>(A, X.base + sz * min(kc + q, ke), L[q]);
+            load_fields<T, IS_E, NPM, CP>(A, X.base + sz * min(kc + q, ke), X.dx, X.dy, L[q]);
 #pragma unroll
         for (int q = 0; q < U; ++q) {
             if (kc + q > ke)
@@ -1096,7 +1112,7 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
 template <typename T>
 struct RingArgs {
     const T* u[6];  // E then H components (offset to lattice index 0)
-    int pitch, plane;
+    int pitch, plane, tail0;
     int nunits;
     int64_t first[MAXU * 2 + 1];  // unit q's points are gather threads first[q] .. first[q+1]-1
     DftDev units[MAXU * 2];
@@ -1122,7 +1138,7 @@ __global__ void __launch_bounds__(256) k_dft_gather(const RingArgs<T> A)
     const int r = p / U.dims[0];
     const int jj = r % U.dims[1];
     const int kk = r / U.dims[1];
-    const int o = (U.box[0][0] + ii) + A.pitch * (U.box[1][0] + jj) + A.plane * (U.box[2][0] + kk);
+    const int o = int(lat_off(A.pitch, A.plane, A.tail0, U.box[0][0] + ii, U.box[1][0] + jj, U.box[2][0] + kk));
     const T* u = A.u[A.half[q] * 3 + U.comp];
     static_cast<T*>(U.ring)[int64_t(A.slot) * U.npts + p] = u[o];
 }
@@ -1227,6 +1243,7 @@ template <typename T>
 struct FaceArgs {
     T* u[3];
     int64_t pitch, plane;
+    int tail0;
     int nf;
     FaceDev f[MAXF];
 };
@@ -1244,9 +1261,9 @@ __global__ void k_faces(const FaceArgs<T> A)
     c[a0] = t0;
     c[a1] = t1;
     c[F.axis] = F.src;
-    const int64_t s = c[0] + A.pitch * c[1] + A.plane * int64_t(c[2]);
+    const int64_t s = lat_off(A.pitch, A.plane, A.tail0, c[0], c[1], c[2]);
     c[F.axis] = F.dst;
-    const int64_t d = c[0] + A.pitch * c[1] + A.plane * int64_t(c[2]);
+    const int64_t d = lat_off(A.pitch, A.plane, A.tail0, c[0], c[1], c[2]);
     T* u = A.u[F.comp % 3];
     u[d] = T(F.mult * double(u[s]));
 }
@@ -2297,6 +2314,7 @@ HalfArgs<T> EngineImpl::half_args(int h)
         A.s[c] = static_cast<const T*>(field[(1 - h) * 3 + c]);
     }
     A.pitch = int(L.pitch);
+    A.tail0 = int(std::min<int64_t>(L.tail0, Layout::NO_TAIL));
     A.plane = int(L.plane);
     for (int a = 0; a < 3; ++a) {
         A.n[a] = L.n[a];
@@ -2555,6 +2573,7 @@ void EngineImpl::post_half(int h)
         for (int c = 0; c < 3; ++c)
             F.u[c] = static_cast<T*>(field[h * 3 + c]) + S->L.origin;
         F.pitch = S->L.pitch;
+        F.tail0 = S->L.tail0;
         F.plane = S->L.plane;
         F.nf = int(faces[h].size());
         int m0 = 1, m1 = 1;
@@ -2668,6 +2687,10 @@ void EngineImpl::build_fused_tiles()
             }
             x = y + 1;
         }
+        // the fused kernel addresses rows without the packed last line: its box and
+        // the one-point rims around it stay below the packed x indices
+        if (a == 0 && L.tail0 != Layout::NO_TAIL)
+            best1 = std::min(best1, L.tail0 - 2);
         // round inwards to whole tiles (a run that reaches the grid's end keeps it)
         const int first = A.dlo[a] + (best0 - A.dlo[a] + step[a] - 1) / step[a] * step[a];
         const int last = best1 == A.dhi[a] ? A.dhi[a] : A.dlo[a] + (best1 + 1 - A.dlo[a]) / step[a] * step[a] - 1;
@@ -2861,6 +2884,7 @@ RingArgs<T> EngineImpl::ring_args()
     for (int c = 0; c < 6; ++c)
         R.u[c] = static_cast<const T*>(field[c]) + S->L.origin;
     R.pitch = int(S->L.pitch);
+    R.tail0 = S->L.tail0;
     R.plane = int(S->L.plane);
     R.nunits = int(ring_units.size());
     R.first[0] = 0;

@@ -512,7 +512,17 @@ static void make_layout(Setup& S)
     L.plane = L.pitch * L.rows;
     L.nplanes = L.n[2] + 2;
     L.alloc = L.plane * L.nplanes;
-    L.origin = L.plane;
+    // the first row of the allocation is row j = -1 of plane k = -1, so every
+    // index with i, j, k >= -1 (and the packed last line of that plane's first
+    // rows) lies inside it
+    L.origin = L.plane + L.pitch;
+    // pack the last line of every row when it would hold only its last two indices
+    // (grids of 32k cells along x); FFIP_PACK_TAIL=0 keeps one line per row
+    const char* pk = std::getenv("FFIP_PACK_TAIL");
+    const bool pack = !(pk && std::atoi(pk) == 0);
+    L.tail0 = Layout::NO_TAIL;
+    if (pack && L.n[0] > 2 && (L.n[0] - 2) % 32 == 0 && L.pitch == L.n[0] - 2 + 32)
+        L.tail0 = L.n[0] - 2;
     for (int c = 0; c < 6; ++c)
         for (int a = 0; a < 3; ++a) {
             int p = comp_parity(c, a);
@@ -985,8 +995,14 @@ static void set_materials(const std::vector<std::unique_ptr<Geometry>>& geoms, S
             if (e0 <= 0 || e1 <= 0 || e2 <= 0)
                 continue;
             parallel_planes(lo2, lo2 + e2 - 1, [&](int k) {
-                for (int j = lo1; j < lo1 + e1; ++j)
-                    std::memset(ids + L.index(lo0, j, k), bg_id, size_t(e0));
+                // contiguous up to the packed indices, which are set one by one
+                const int e0c = std::max(0, std::min(e0, L.tail0 - lo0));
+                for (int j = lo1; j < lo1 + e1; ++j) {
+                    if (e0c > 0)
+                        std::memset(ids + L.index(lo0, j, k), bg_id, size_t(e0c));
+                    for (int i = lo0 + e0c; i < lo0 + e0; ++i)
+                        ids[L.index(i, j, k)] = uint8_t(bg_id);
+                }
             });
             const int b0 = ob[cc][0][0], b1 = ob[cc][1][0], b2 = ob[cc][2][0];
             const int n0 = ob[cc][0][1] - b0 + 1, n1 = ob[cc][1][1] - b1 + 1;

@@ -130,16 +130,35 @@ struct AbsMedium {
 
 enum Bc : int { BC_NONE = 0, BC_PERIOD = 1, BC_PEC = 2, BC_PMC = 3 };
 
+// Rows whose last two x indices share one packed line (see Layout::tail0).
+constexpr int TAIL_GROUP = 15;
+
 struct Layout {
     int P1[3], P2[3];  // doubled-coordinate domain (sim_p1, sim_p2)
     int M[3];          // ix = (x + M) >> 1
     int n[3];          // index extent per axis, ghosts included
     int64_t pitch, rows, plane, nplanes, alloc, origin;
     int lo[6][3], hi[6][3];  // real points of each component (inclusive)
+    // Rows are `pitch` elements (a whole number of 128-byte lines of FP32).  When
+    // the last two x indices of a row (its last real point and the ghost after it)
+    // would sit alone in the row's last line, tail0 is the first of them and
+    // they are packed instead: those two indices of the TAIL_GROUP rows
+    // j = 15g - 1 .. 15g + 13 of a plane are stored side by side in the last line
+    // of row 15g - 1, so one line serves 15 rows.  Otherwise tail0 = NO_TAIL.
+    static constexpr int NO_TAIL = 1 << 30;
+    int tail0 = NO_TAIL;
 
     int ix(int a, int x) const { return (x + M[a]) >> 1; }
     int coord(int a, int i, int parity) const { return 2 * i - M[a] + (1 - parity); }
-    int64_t index(int i, int j, int k) const { return origin + i + pitch * j + plane * k; }
+    // element of lattice index (i, j, k) in every per-cell array; j >= -1, k >= -1
+    // (the kernels' lat_off is the same map)
+    int64_t index(int i, int j, int k) const
+    {
+        if (i < tail0)
+            return origin + i + pitch * j + plane * k;
+        const int m = (j + 1) % TAIL_GROUP;
+        return origin + tail0 + 2 * m + (i - tail0) + pitch * (j - m) + plane * k;
+    }
     int64_t count(int comp) const;
 };
 

@@ -137,6 +137,12 @@ def _tail_cases():
     cfg["fields output"] = [W.dft((0, 0, 0), (32, 4, 0), f, comp, "t%d" % q)
                             for q, comp in enumerate(["Ey", "Dy", "Bz", "Hz", "Dx", "Hx"])]
     c["magnetic32"] = (cfg, None, [((15.5, 0.5, 0.0), "Ey"), ((16.0, 1.0, 0.5), "Hz")])
+    # ghost faces on the packed last line: a mirror plane at the low x face with
+    # periodic y (x from 0 to 64: 32 cells), and periodic x and y
+    c["symmetry64"] = (_widened(CASES["symmetry"][0], 64), None, [((15.5, 1.0, 0.5), "Ey")])
+    cfg = dict(CASES["periodic"][0])
+    cfg["size"] = [32.0, 32.0] + [float(v) for v in cfg["size"][2:]]
+    c["periodic32"] = (cfg, None, [])
     return c
 
 
@@ -147,9 +153,10 @@ TAIL_CASES = _tail_cases()
 def test_power_of_two_grids_match_oracle(name):
     """Grids of 2^k cells along x have 2^k + 1 lattice points there, so the last
     32-point tile column holds one point per row (the layout of the benchmark
-    grids): FP64 against the oracle, with poles in both polarization layouts,
-    per-point media, magnetic media and flux-density monitors reaching the
-    boundary faces."""
+    grids), and the last two x indices of 15 rows share a packed line: FP64
+    against the oracle, with poles in both polarization layouts, per-point
+    media, magnetic media, flux-density monitors reaching the boundary faces,
+    mirror and periodic ghost faces."""
     cfg, ds, probes = TAIL_CASES[name]
     _compare(cfg, ds, probes, steps=150, label=name)
 
