@@ -12,6 +12,7 @@
 // every design it proposes is re-judged by the full FDTD + adjoint solve.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -132,6 +133,16 @@ struct ConvGeom {
     int out[3];
     int s[3], d[3];
 };
+
+// zero the M x N block of C (rows ldc apart) before split-K partial tiles are
+// added onto it: a kernel, not hipMemset2DAsync, so that a caller may capture the
+// GEMM in a HIP graph
+__global__ void k_zero_block(float* C, int64_t M, int64_t N, int64_t ldc)
+{
+    const int64_t total = M * N;
+    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += int64_t(gridDim.x) * blockDim.x)
+        C[(t / N) * ldc + t % N] = 0.f;
+}
 
 __global__ void k_im2col(ConvGeom g, const __bf16* __restrict__ X, __bf16* __restrict__ col)
 {
@@ -359,8 +370,11 @@ void sg_gemm(int ta, int tb, int64_t M, int64_t N, int64_t K, const void* A, int
     dim3 grid(unsigned((M + GBM - 1) / GBM), unsigned((N + GBN - 1) / GBN), unsigned(nz < 1 ? 1 : nz));
     const __bf16* a = static_cast<const __bf16*>(A);
     const __bf16* b = static_cast<const __bf16*>(B);
-    if (nz > 1 && !accumulate)  // partial tiles are added atomically onto C
-        SG_OK(hipMemset2DAsync(C, size_t(ldc) * sizeof(float), 0, size_t(N) * sizeof(float), size_t(M), st));
+    if (nz > 1 && !accumulate) {  // partial tiles are added atomically onto C
+        const int64_t blocks = std::min<int64_t>((M * N + 255) / 256, 4096);
+        k_zero_block<<<unsigned(blocks), 256, 0, st>>>(C, M, N, ldc);
+        SG_OK(hipGetLastError());
+    }
     const int m = int(M), n = int(N), k = int(K), la = int(lda), lb = int(ldb), lc = int(ldc), kc = int(kchunk);
     if (!ta && !tb)
         k_gemm<false, false><<<grid, 256, 0, st>>>(m, n, k, a, la, b, lb, C, lc, accumulate, kc);

@@ -74,6 +74,39 @@ def test_gemm_matches_float64(ta, tb, ksplit):
     assert err < 1e-5, err
 
 
+@pytest.mark.gpu
+def test_split_k_gemm_replays_in_a_graph():
+    """The split-K GEMM clears its output with a kernel, so a caller can capture
+    it in a HIP graph: every replay, with eager work between replays, gives the
+    eager result."""
+    import torch
+    rng = np.random.default_rng(17)
+    M, N, K = 200, 40, 100
+    B = _bf16_round(rng.standard_normal((K, N)))
+    Ad = torch.tensor(_bf16_round(rng.standard_normal((M, K))), dtype=torch.bfloat16, device="cuda")
+    Bd = torch.tensor(B, dtype=torch.bfloat16, device="cuda")
+    Cd = torch.zeros((M, N), dtype=torch.float32, device="cuda")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        SG.gemm(Ad, Bd, Cd, M, N, K, ksplit=3)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        SG.gemm(Ad, Bd, Cd, M, N, K, ksplit=3)
+    for rep in range(3):
+        A = _bf16_round(rng.standard_normal((M, K)))
+        Ad.copy_(torch.tensor(A, dtype=torch.bfloat16))
+        Cd.fill_(float(rep + 3))  # stale values the replay must clear
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        ref = A @ B
+        err = np.linalg.norm(Cd.cpu().numpy() - ref) / np.linalg.norm(ref)
+        assert err < 1e-5, (rep, err)
+
+
 def _torch_conv_ref(x, w, b, stride, dil, relu):
     """float64 CPU reference: x [B, z, y, x, C], w [27*C, Co]."""
     import torch
