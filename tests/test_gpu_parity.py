@@ -161,6 +161,14 @@ def test_power_of_two_grids_match_oracle(name):
     _compare(cfg, ds, probes, steps=150, label=name)
 
 
+@pytest.mark.parametrize("name", ["voxels32", "magnetic32", "periodic32", "spheres64"])
+def test_power_of_two_grids_fp32_storage(name):
+    """The packed last line in FP32 storage (FP64 arithmetic) against the FP64
+    oracle, within the FP32-storage tolerance of the small-grid cases."""
+    cfg, ds, probes = TAIL_CASES[name]
+    _compare(dict(cfg, precision="fp32"), ds, probes, steps=150, tol=1e-5, label=name + "-fp32")
+
+
 @pytest.mark.parametrize("split", ["0", "1"])
 def test_dispersive_default_medium(split, monkeypatch):
     """A dispersive default medium with pole-free boxes in it: points outside the
