@@ -91,7 +91,16 @@ def _cases():
         "voxels": (_widen(W.voxel_mix(n=72, voxel=4, pml_cells=5, nfreq=3, nplanes=3, precision="fp64",
                                       source_offset=8), (72, 24, 28)), None),
         "design": d,
+        # 64 cells along x without CPML there: the fused box would reach the
+        # packed last line of the rows and stops one tile short of it
+        "tail64": _tail64(),
     }
+
+
+def _tail64():
+    cfg = _widen(W.vacuum_dipole(n=64, steps=60, pml_cells=5, plane_z=4), (64, 26, 30))
+    cfg["PML"] = [dict(W.pml(5), direction="y"), dict(W.pml(5), direction="z")]
+    return cfg, None
 
 
 FUSED_CASES = _cases()
@@ -112,7 +121,7 @@ def test_fused_equals_half_step_kernels(name, prec, kernel, monkeypatch):
         _compare(a, b, 1e-13 if prec == "fp64" else 1e-6, "%s %s %s lz=%s" % (name, prec, kernel, lzf))
 
 
-@pytest.mark.parametrize("name", ["spheres", "voxels", "design"])
+@pytest.mark.parametrize("name", ["spheres", "voxels", "design", "tail64"])
 @pytest.mark.parametrize("kernel", sorted(KERNELS))
 def test_fused_fp64_matches_oracle(name, kernel, monkeypatch):
     cfg, ds = FUSED_CASES[name]

@@ -161,7 +161,7 @@ def test_power_of_two_grids_match_oracle(name):
     _compare(cfg, ds, probes, steps=150, label=name)
 
 
-@pytest.mark.parametrize("name", ["voxels32", "magnetic32", "periodic32", "spheres64"])
+@pytest.mark.parametrize("name", sorted(TAIL_CASES))
 def test_power_of_two_grids_fp32_storage(name):
     """The packed last line in FP32 storage (FP64 arithmetic) against the FP64
     oracle, within the FP32-storage tolerance of the small-grid cases."""
