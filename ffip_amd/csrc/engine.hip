@@ -738,7 +738,8 @@ __device__ __forceinline__ T h_value(const StepArgs<T>& S, double d1, double d2,
 // per workgroup) rather than by the register budget.  The loading wavefront
 // issues nothing but these loads, so waiting for a plane is a static count of
 // outstanding loads (vmcnt), followed by the workgroup barrier that also orders
-// the computing wavefronts' LDS traffic.
+// the computing wavefronts' LDS traffic.  It addresses rows as origin + i +
+// pitch * j: its box and rims stay below the packed last line (Layout::tail0).
 //
 // Ring slot layout, in dwords: for c = 0..2 the E^n region [BY+2][BX+2] (x from
 // i0-1, y from j0-1), then for c = 0..2 the H region [BY+1][BX+1] (x from i0, y
