@@ -1885,7 +1885,18 @@ void EngineImpl::build()
     // design-box electric kernel needs 102 VGPRs (4 waves per SIMD) instead of 75
     // (6), and the 200x200x100 design gradient's electric half step took
     // 49.2 us per step instead of 44.2.
+    // Segments of 12 planes on grids that still give at least 24576 tiles (96 per
+    // CU): fewer planes are re-read at segment ends.  Measured: 512^3 voxel mix
+    // 3.837 -> 3.775 ms per step (10 planes 3.813, 14 planes 3.775), 640^3 with
+    // 8 x 64-frequency planes 7.467 -> 7.394 ms; on the 256^3 spheres (6534
+    // tiles at 12 planes) 0.391 -> 0.406 ms, and the 200x200x100 design grid is
+    // best at 8.
     lz = 8;
+    {
+        const int64_t tiles12 = int64_t((L.n[0] + BX - 1) / BX) * ((L.n[1] + BY - 1) / BY) * ((L.n[2] + 11) / 12);
+        if (tiles12 >= 24576)
+            lz = 12;
+    }
     unroll = fp32 ? 2 : 1;
     unroll_mat = fp32 ? 2 : 1;
     unroll_shell = fp32 ? 0 : 1;
@@ -2284,9 +2295,9 @@ void EngineImpl::build()
             // a multiple of the half-step kernels' segment, so that their shell
             // tiles never straddle a fused segment
             lzf = lz;
-            for (int c : {32, 16, 8})
-                if (c % lz == 0 && int64_t(cols) * ((nz + c - 1) / c) >= 4096) {
-                    lzf = c;
+            for (int m : {4, 2})
+                if (int64_t(cols) * ((nz + m * lz - 1) / (m * lz)) >= 4096) {
+                    lzf = m * lz;
                     break;
                 }
             if (const char* e = std::getenv("FFIP_LZF"))
