@@ -169,6 +169,15 @@ def test_power_of_two_grids_fp32_storage(name):
     _compare(dict(cfg, precision="fp32"), ds, probes, steps=150, tol=1e-5, label=name + "-fp32")
 
 
+@pytest.mark.parametrize("name", ["voxels32", "spheres64", "magnetic32"])
+def test_twelve_plane_segments(name, monkeypatch):
+    """The 12-plane z segments that large grids use (and the polarization plan
+    ordered for them), forced on grids the oracle runs: FP64 against the oracle."""
+    monkeypatch.setenv("FFIP_LZ", "12")
+    cfg, ds, probes = TAIL_CASES[name]
+    _compare(cfg, ds, probes, steps=150, label=name + "-lz12")
+
+
 @pytest.mark.parametrize("split", ["0", "1"])
 def test_dispersive_default_medium(split, monkeypatch):
     """A dispersive default medium with pole-free boxes in it: points outside the
