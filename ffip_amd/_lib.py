@@ -67,6 +67,9 @@ _SIGS = {
     "ffip_sg_gemm": ([ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                       ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
                       ctypes.c_int, ctypes.c_void_p], ctypes.c_int),
+    "ffip_sg_gemm_ws": ([ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                         ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                         ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p], ctypes.c_int),
     "ffip_sg_im2col": ([ctypes.c_void_p] * 4, ctypes.c_int),
     "ffip_sg_col2im": ([ctypes.c_void_p] * 4, ctypes.c_int),
     "ffip_sg_bias_act": ([ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,

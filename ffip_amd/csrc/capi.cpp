@@ -706,6 +706,18 @@ int ffip_sg_gemm(int ta, int tb, int64_t M, int64_t N, int64_t K, const void* A,
     return guard([&] { sg_gemm(ta, tb, M, N, K, A, lda, B, ldb, C, ldc, accumulate, ksplit, stream); });
 }
 
+int ffip_sg_gemm_ws(int ta, int tb, int64_t M, int64_t N, int64_t K, const void* A, int64_t lda, const void* B,
+                    int64_t ldb, float* C, int64_t ldc, int accumulate, int ksplit, float* work, int64_t work_elems,
+                    void* stream)
+{
+    if ((M > 0 && N > 0 && (!C || (K > 0 && (!A || !B)))) || lda < 0 || ldb < 0 || ldc < N || work_elems < 0 ||
+        (work_elems > 0 && !work))
+        return fail(FFIP_EINVAL, "bad GEMM arguments");
+    return guard([&] {
+        sg_gemm(ta, tb, M, N, K, A, lda, B, ldb, C, ldc, accumulate, ksplit, stream, work, work_elems);
+    });
+}
+
 int ffip_sg_im2col(const int64_t geom[11], const void* X, void* col, void* stream)
 {
     if (!geom || !X || !col)

@@ -36,14 +36,16 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // with leading dimension ldc.  TA: A is stored K x M (A[k*lda + m]); TB: B is
 // stored N x K (B[n*ldb + k]).  A workgroup of 4 wavefronts owns a 128 x 32 tile
 // of C, each wavefront a 32 x 32 MFMA tile; K advances 32 at a time through LDS.
-// With ksplit > 1 the K range is split over gridDim.z and partial tiles are
-// added atomically (C must hold the initial value).
+// With ksplit > 1 the K range is split over gridDim.z.  Partial tiles are added
+// atomically onto C (C must hold the initial value), or, when `part` is given,
+// stored to part[z][M][N] for k_splitk_sum to add up in z order (the same bits
+// on every run).
 static constexpr int GBM = 128, GBN = 32, GBK = 32, GPAD = 8;
 
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const __bf16* __restrict__ A, int lda,
                                               const __bf16* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
-                                              int accumulate, int kchunk)
+                                              int accumulate, int kchunk, float* __restrict__ part)
 {
     __shared__ __bf16 As[GBM][GBK + GPAD];
     __shared__ __bf16 Bs[GBN][GBK + GPAD];  // stored [n][k]
@@ -115,7 +117,9 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, const __bf16*
         const int n = n0 + (lane & 31);
         if (m < M && n < N) {
             float* c = C + int64_t(m) * ldc + n;
-            if (gridDim.z > 1)
+            if (gridDim.z > 1 && part)
+                part[(int64_t(blockIdx.z) * M + m) * N + n] = acc[j];
+            else if (gridDim.z > 1)
                 atomicAdd(c, acc[j]);
             else
                 *c = accumulate ? *c + acc[j] : acc[j];
@@ -142,6 +146,20 @@ __global__ void k_zero_block(float* C, int64_t M, int64_t N, int64_t ldc)
     const int64_t total = M * N;
     for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += int64_t(gridDim.x) * blockDim.x)
         C[(t / N) * ldc + t % N] = 0.f;
+}
+
+// C (+)= the nz partial tiles part[z][M][N], added in z order
+__global__ void k_splitk_sum(const float* __restrict__ part, int64_t nz, int64_t M, int64_t N, float* C, int64_t ldc,
+                             int accumulate)
+{
+    const int64_t total = M * N;
+    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += int64_t(gridDim.x) * blockDim.x) {
+        float* c = C + (t / N) * ldc + t % N;
+        float v = accumulate ? *c : 0.f;
+        for (int64_t z = 0; z < nz; ++z)
+            v += part[z * total + t];
+        *c = v;
+    }
 }
 
 __global__ void k_im2col(ConvGeom g, const __bf16* __restrict__ X, __bf16* __restrict__ col)
@@ -353,7 +371,7 @@ static ConvGeom geom_of(const int64_t g[11])
 }
 
 void sg_gemm(int ta, int tb, int64_t M, int64_t N, int64_t K, const void* A, int64_t lda, const void* B, int64_t ldb,
-             float* C, int64_t ldc, int accumulate, int ksplit, void* stream)
+             float* C, int64_t ldc, int accumulate, int ksplit, void* stream, float* work, int64_t work_elems)
 {
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (M < 0 || N < 0 || K < 0 || M >= (int64_t(1) << 31) || N >= (int64_t(1) << 31) || K >= (int64_t(1) << 31))
@@ -370,21 +388,30 @@ void sg_gemm(int ta, int tb, int64_t M, int64_t N, int64_t K, const void* A, int
     dim3 grid(unsigned((M + GBM - 1) / GBM), unsigned((N + GBN - 1) / GBN), unsigned(nz < 1 ? 1 : nz));
     const __bf16* a = static_cast<const __bf16*>(A);
     const __bf16* b = static_cast<const __bf16*>(B);
-    if (nz > 1 && !accumulate) {  // partial tiles are added atomically onto C
-        const int64_t blocks = std::min<int64_t>((M * N + 255) / 256, 4096);
+    const int64_t blocks = std::min<int64_t>((M * N + 255) / 256, 4096);
+    float* part = nullptr;
+    if (nz > 1 && work) {  // partial tiles to the workspace, added up in order below
+        if (work_elems < nz * M * N)
+            throw std::runtime_error("GEMM workspace too small: " + std::to_string(nz * M * N) + " floats needed");
+        part = work;
+    } else if (nz > 1 && !accumulate) {  // partial tiles are added atomically onto C
         k_zero_block<<<unsigned(blocks), 256, 0, st>>>(C, M, N, ldc);
         SG_OK(hipGetLastError());
     }
     const int m = int(M), n = int(N), k = int(K), la = int(lda), lb = int(ldb), lc = int(ldc), kc = int(kchunk);
     if (!ta && !tb)
-        k_gemm<false, false><<<grid, 256, 0, st>>>(m, n, k, a, la, b, lb, C, lc, accumulate, kc);
+        k_gemm<false, false><<<grid, 256, 0, st>>>(m, n, k, a, la, b, lb, C, lc, accumulate, kc, part);
     else if (ta && !tb)
-        k_gemm<true, false><<<grid, 256, 0, st>>>(m, n, k, a, la, b, lb, C, lc, accumulate, kc);
+        k_gemm<true, false><<<grid, 256, 0, st>>>(m, n, k, a, la, b, lb, C, lc, accumulate, kc, part);
     else if (!ta && tb)
-        k_gemm<false, true><<<grid, 256, 0, st>>>(m, n, k, a, la, b, lb, C, lc, accumulate, kc);
+        k_gemm<false, true><<<grid, 256, 0, st>>>(m, n, k, a, la, b, lb, C, lc, accumulate, kc, part);
     else
-        k_gemm<true, true><<<grid, 256, 0, st>>>(m, n, k, a, la, b, lb, C, lc, accumulate, kc);
+        k_gemm<true, true><<<grid, 256, 0, st>>>(m, n, k, a, la, b, lb, C, lc, accumulate, kc, part);
     SG_OK(hipGetLastError());
+    if (part) {
+        k_splitk_sum<<<unsigned(blocks), 256, 0, st>>>(part, nz, M, N, C, ldc, accumulate);
+        SG_OK(hipGetLastError());
+    }
 }
 
 void sg_im2col(const int64_t geom[11], const void* X, void* col, void* stream)

@@ -7,9 +7,11 @@
 namespace ffip_amd {
 
 // C (+)= op(A) op(B), A and B bf16, C FP32 (see k_gemm); ksplit > 1 splits K
-// over workgroups (partial tiles added atomically).
+// over workgroups, their partial tiles added atomically or, with a workspace of
+// at least ksplit * M * N floats, in a fixed order.
 void sg_gemm(int ta, int tb, int64_t M, int64_t N, int64_t K, const void* A, int64_t lda, const void* B, int64_t ldb,
-             float* C, int64_t ldc, int accumulate, int ksplit, void* stream);
+             float* C, int64_t ldc, int accumulate, int ksplit, void* stream, float* work = nullptr,
+             int64_t work_elems = 0);
 // geom = {batch, C, in_z, in_y, in_x, s_z, s_y, s_x, d_z, d_y, d_x}
 void sg_im2col(const int64_t geom[11], const void* X, void* col, void* stream);
 void sg_col2im(const int64_t geom[11], const float* dcol, float* dX, void* stream);

@@ -54,13 +54,20 @@ def _stream():
 
 
 # ---------------------------------------------------------------- kernel calls
-def gemm(A, B, C, M, N, K, ta=False, tb=False, lda=None, ldb=None, ldc=None, accumulate=False, ksplit=1):
-    """C (+)= op(A) op(B) on the matrix cores (A, B bf16 tensors, C FP32)."""
+def gemm(A, B, C, M, N, K, ta=False, tb=False, lda=None, ldb=None, ldc=None, accumulate=False, ksplit=1,
+         work=None):
+    """C (+)= op(A) op(B) on the matrix cores (A, B bf16 tensors, C FP32).  With
+    a FP32 workspace `work` of at least ksplit * M * N elements, the split-K
+    partial tiles are added in a fixed order (the same bits on every run)."""
     lda = lda if lda is not None else (M if ta else K)
     ldb = ldb if ldb is not None else (K if tb else N)
     ldc = ldc if ldc is not None else N
-    _check(_lib.lib().ffip_sg_gemm(int(ta), int(tb), M, N, K, _vp(A), lda, _vp(B), ldb, _vp(C), ldc,
-                                   int(accumulate), int(ksplit), _stream()))
+    if work is None:
+        _check(_lib.lib().ffip_sg_gemm(int(ta), int(tb), M, N, K, _vp(A), lda, _vp(B), ldb, _vp(C), ldc,
+                                       int(accumulate), int(ksplit), _stream()))
+    else:
+        _check(_lib.lib().ffip_sg_gemm_ws(int(ta), int(tb), M, N, K, _vp(A), lda, _vp(B), ldb, _vp(C), ldc,
+                                          int(accumulate), int(ksplit), _vp(work), int(work.numel()), _stream()))
 
 
 def conv_out(shape, stride):
@@ -84,6 +91,7 @@ class Conv3:
         self.m = [torch.zeros_like(self.w), torch.zeros_like(self.b)]
         self.v = [torch.zeros_like(self.w), torch.zeros_like(self.b)]
         self.cache = None
+        self.work = None  # split-K workspace of the weight gradient
 
     def geom(self, batch, shape):
         return [batch, self.cin] + list(shape) + list(self.stride) + list(self.dil)
@@ -116,8 +124,12 @@ class Conv3:
         _check(_lib.lib().ffip_sg_act_back(M * self.cout, _vp(dA), _vp(Z), int(self.relu), _vp(dZb), _vp(dZf),
                                            _stream()))
         _check(_lib.lib().ffip_sg_colsum(M, self.cout, _vp(dZf), _vp(self.gb), 0, _stream()))
-        # weight gradient col^T dZ: long K (= positions), split over workgroups
-        gemm(col, dZb, self.gw, K, self.cout, M, ta=True, lda=K, ldb=self.cout, ksplit=max(1, M // 8192))
+        # weight gradient col^T dZ: long K (= positions), split over workgroups,
+        # the partial tiles added in a fixed order so that training is repeatable
+        ks = max(1, M // 8192)
+        if self.work is None or self.work.numel() < ks * K * self.cout:
+            self.work = torch.empty(ks * K * self.cout, dtype=torch.float32, device="cuda")
+        gemm(col, dZb, self.gw, K, self.cout, M, ta=True, lda=K, ldb=self.cout, ksplit=ks, work=self.work)
         if not need_dx:
             return None
         dcol = torch.empty((M, K), dtype=torch.float32, device="cuda")

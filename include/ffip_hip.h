@@ -186,6 +186,14 @@ int ffip_to_convolve_noalloc(const double *in, int64_t n0, int64_t n1, const dou
 int ffip_sg_gemm(int ta, int tb, int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B,
                  int64_t ldb, float *C, int64_t ldc, int accumulate, int ksplit, void *stream);
 
+/* ffip_sg_gemm whose split-K partial tiles go to a caller-owned device
+ * workspace of work_elems floats (at least ksplit * M * N) and are added up in a
+ * fixed order, so that repeated calls give the same bits (the atomic adds of
+ * ffip_sg_gemm do not).  No reference counterpart (surrogate kernel). */
+int ffip_sg_gemm_ws(int ta, int tb, int64_t M, int64_t N, int64_t K, const void *A, int64_t lda, const void *B,
+                    int64_t ldb, float *C, int64_t ldc, int accumulate, int ksplit, float *work, int64_t work_elems,
+                    void *stream);
+
 /* 3x3x3 convolution taps of a channels-last bf16 activation [batch][z][y][x][C]
  * with stride s and dilation d per axis (padding d, output extent
  * (in - 1) / s + 1): col[(b, oz, oy, ox)][(kz, ky, kx, c)].

@@ -107,6 +107,51 @@ def test_split_k_gemm_replays_in_a_graph():
         assert err < 1e-5, (rep, err)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0)])
+@pytest.mark.parametrize("accumulate", [0, 1])
+def test_gemm_workspace_matches_float64_and_repeats(ta, tb, accumulate):
+    """Split-K with a workspace: the partial tiles are added in a fixed order, so
+    two calls give the same bits, and the product equals float64 to 1e-5."""
+    import torch
+    rng = np.random.default_rng(11 + ta + 2 * accumulate)
+    M, N, K, ks = 200, 40, 1000, 7
+    A = _bf16_round(rng.standard_normal((K, M) if ta else (M, K)))
+    B = _bf16_round(rng.standard_normal((K, N)))
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    Ad = torch.tensor(A, dtype=torch.bfloat16, device="cuda")
+    Bd = torch.tensor(B, dtype=torch.bfloat16, device="cuda")
+    work = torch.empty(ks * M * N, dtype=torch.float32, device="cuda")
+    out = []
+    for _ in range(2):
+        Cd = torch.tensor(C0, device="cuda")
+        SG.gemm(Ad, Bd, Cd, M, N, K, ta=bool(ta), accumulate=bool(accumulate), ksplit=ks, work=work)
+        out.append(Cd.cpu().numpy())
+    assert np.array_equal(out[0], out[1])
+    ref = (A.T if ta else A) @ B + (C0 if accumulate else 0)
+    err = np.linalg.norm(out[0] - ref) / np.linalg.norm(ref)
+    assert err < 1e-5, err
+    small = torch.empty(M * N, dtype=torch.float32, device="cuda")
+    with pytest.raises(_lib.FfipError):
+        SG.gemm(Ad, Bd, torch.tensor(C0, device="cuda"), M, N, K, ta=bool(ta), ksplit=ks, work=small)
+
+
+@pytest.mark.gpu
+def test_training_repeats_bit_for_bit():
+    """Two trainings from the same seed give the same weights: the weight
+    gradients' split-K tiles are added in a fixed order."""
+    rng = np.random.default_rng(3)
+    rhos = rng.random((8, 5, 48, 48)).astype(np.float32)
+    t = np.stack([np.stack([r[:, 10:19, 10:19].mean(0) * (k + 1) for k in range(4)], -1) for r in rhos])
+    weights = []
+    for _ in range(2):
+        net = SG.Surrogate(seed=1, shape=(5, 48, 48), crop=(10, 10, 9, 9), width=16)
+        net.fit(rhos, t, epochs=2, batch=8, lr=3e-3)
+        weights.append([L.w.cpu().numpy().copy() for L in net.layers])
+    for a, b in zip(*weights):
+        assert np.array_equal(a, b)
+
+
 def _torch_conv_ref(x, w, b, stride, dil, relu):
     """float64 CPU reference: x [B, z, y, x, C], w [27*C, Co]."""
     import torch
