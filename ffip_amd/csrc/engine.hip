@@ -93,9 +93,10 @@ struct PmlDev {
     int W;
 };
 
-struct MatEntry {
-    double inv, eps;
+struct alignas(16) MatEntry {
+    double inv;
     int np, p0;
+    double eps, pad;
 };
 
 template <typename T>
@@ -120,7 +121,7 @@ struct HalfArgs {
     uint32_t bg_word;
     const MatEntry* mats;
     int nmats;
-    const double* coef;  // [npoles][3] = (c1, c2, sigma*c3) of the table's poles
+    const double* coef;  // [npoles][4] = (c1, c2, sigma*c3, 0) of the table's poles
     int npoles;
     int d_box[3][3][2];
     int d_np[3];
@@ -438,9 +439,10 @@ __device__ __forceinline__ void update_comp(const HalfArgs<T>& A, const MatEntry
 #pragma unroll
             for (int s = 0; s < NPM; ++s) {
                 if (s < m.np) {
-                    const double* cf = spc + 3 * (m.p0 + s);
+                    const double* cf = spc + 4 * (m.p0 + s);
+                    const double2 c01 = *reinterpret_cast<const double2*>(cf);
                     const double pc = double(L.pc[C][s]);
-                    const double pn = cf[0] * pc + cf[1] * double(L.pp[C][s]) + cf[2] * e_old;
+                    const double pn = c01.x * pc + c01.y * double(L.pp[C][s]) + cf[2] * e_old;
                     const T pt = T(pn);
                     *pole_ptr<T, NPM, CP>(A, L, Z, C, s) = pt;
                     dP += double(pt) - pc;
@@ -509,16 +511,16 @@ template <typename T, bool IS_E, bool MAT, int NPM, bool CP, int U, bool PML, bo
 __global__ void __launch_bounds__(BX * BY) k_half(const HalfArgs<T> A)
 {
     // material table and pole coefficients, staged in LDS (sized at launch:
-    // 24 bytes per table entry and per pole)
-    extern __shared__ double lds_tab[];
+    // 32 bytes per table entry and per pole, read with 16-byte loads)
+    extern __shared__ __align__(16) double lds_tab[];
     MatEntry* smat = reinterpret_cast<MatEntry*>(lds_tab);
-    double* spc = lds_tab + 3 * A.nmats;
+    double* spc = lds_tab + 4 * A.nmats;
     if (MAT) {
         const int tid = int(threadIdx.x + BX * threadIdx.y);
         for (int q = tid; q < A.nmats; q += BX * BY)
             smat[q] = A.mats[q];
         if (NPM > 0)
-            for (int q = tid; q < 3 * A.npoles; q += BX * BY)
+            for (int q = tid; q < 4 * A.npoles; q += BX * BY)
                 spc[q] = A.coef[q];
         __syncthreads();
     }
@@ -788,9 +790,9 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
 {
     using RG = FRing<T, D>;
     const HalfArgs<T>& A = S.e;
-    extern __shared__ double lds_tab[];
+    extern __shared__ __align__(16) double lds_tab[];
     MatEntry* smat = reinterpret_cast<MatEntry*>(lds_tab);
-    double* spc = lds_tab + 3 * A.nmats;
+    double* spc = lds_tab + 4 * A.nmats;
     __shared__ uint32_t ring[RG::R * RG::SLOT];
     const int tid = int(threadIdx.x);
     const bool loader = tid >= BX * BY;
@@ -798,7 +800,7 @@ __global__ void __launch_bounds__(BX * BY + 64) k_fstep(const StepArgs<T> S)
         for (int q = tid; q < A.nmats; q += BX * BY)
             smat[q] = A.mats[q];
         if (NPM > 0)
-            for (int q = tid; q < 3 * A.npoles; q += BX * BY)
+            for (int q = tid; q < 4 * A.npoles; q += BX * BY)
                 spc[q] = A.coef[q];
         // visible after the first barrier
     }
@@ -1967,7 +1969,7 @@ void EngineImpl::build()
             ids[h] = upload(packed.get(), na);
             std::vector<MatEntry> me(M.table.eps.size());
             for (size_t e = 0; e < me.size(); ++e)
-                me[e] = {M.table.inv_eps[e], M.table.eps[e], M.table.npoles[e], M.table.pole0[e]};
+                me[e] = {M.table.inv_eps[e], M.table.npoles[e], M.table.pole0[e], M.table.eps[e], 0.0};
             nmats[h] = int(me.size());
             mats[h] = upload(me);
             std::vector<double> cf;
@@ -1975,10 +1977,11 @@ void EngineImpl::build()
                 cf.push_back(M.table.c1[p]);
                 cf.push_back(M.table.c2[p]);
                 cf.push_back(M.table.sc3[p]);
+                cf.push_back(0.0);
             }
             npoles[h] = int(M.table.c1.size());
             p_coef[h] = upload(cf);
-            if (size_t(nmats[h] + npoles[h]) * 3 * sizeof(double) > 65536)
+            if (size_t(nmats[h] + npoles[h]) * 4 * sizeof(double) > 65536)
                 throw std::runtime_error("material table too large for LDS");
             for (int c = 0; c < 3; ++c) {
                 DenseRegion& D = M.dense[c];
@@ -2512,7 +2515,7 @@ void EngineImpl::launch_tiles(int h, const HalfArgs<T>& A, const uint32_t* list,
         return;
     const HalfMaterial& M = S->mat[h];
     const bool mat = !M.uniform;
-    const size_t lds = mat ? size_t(A.nmats + A.npoles) * 3 * sizeof(double) : 0;
+    const size_t lds = mat ? size_t(A.nmats + A.npoles) * 4 * sizeof(double) : 0;
     // pole kernels on the CPML tiles of a split half step take unroll_shell planes per batch
     const bool split = fused || !pml_a[h];
     const int u = pml && split && unroll_shell > 0 && mat && S_slots[h] > 0
@@ -2834,7 +2837,7 @@ template <typename T>
 void EngineImpl::step_fused(hipEvent_t* ev)
 {
     StepArgs<T> S0 = step_args<T>();
-    const size_t lds = S->mat[0].uniform ? 0 : size_t(S0.e.nmats + S0.e.npoles) * 3 * sizeof(double);
+    const size_t lds = S->mat[0].uniform ? 0 : size_t(S0.e.nmats + S0.e.npoles) * 4 * sizeof(double);
     if (ev)
         HIP_OK(hipEventRecord(ev[2], stream));
     {
