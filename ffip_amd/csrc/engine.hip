@@ -93,6 +93,8 @@ struct PmlDev {
     int W;
 };
 
+// One medium of the table.  The update reads the first 16 bytes (1/eps, pole
+// count, first pole) in one LDS read; eps is read only for D monitors.
 struct alignas(16) MatEntry {
     double inv;
     int np, p0;
