@@ -100,6 +100,8 @@ struct alignas(16) MatEntry {
     int np, p0;
     double eps, pad;
 };
+static_assert(sizeof(MatEntry) == 32 && alignof(MatEntry) == 16 && offsetof(MatEntry, p0) == 12,
+              "the LDS table layout assumes 32-byte entries with inv, np, p0 in the first 16 bytes");
 
 template <typename T>
 struct HalfArgs {
