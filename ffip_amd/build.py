@@ -22,6 +22,11 @@ ARCH = os.environ.get("FFIP_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["json.cpp", "setup.cpp", "capi.cpp"]
 HIP_SOURCES = ["engine.hip", "surrogate.hip", "filters.hip", "stream.hip"]
+# The step kernels are scheduled for instruction-level parallelism rather than the
+# default occupancy-first order.  Measured on MI355X (same box, ms per step):
+# 512^3 voxel mix FP32 3.743 -> 3.726, FP64 7.690 -> 7.392; 640^3 with DFT planes
+# 7.289 -> 7.251; 256^3 spheres FP32 0.382 -> 0.375, FP64 0.708 -> 0.692.
+HIP_EXTRA = {"engine.hip": ["-mllvm", "--amdgpu-sched-strategy=max-ilp"]}
 # the file drop-in executable (links the library; libhdf5 is loaded at run time)
 EXE_SOURCES = ["run_sim_json.cpp", "hdf5_io.cpp", "json.cpp"]
 
@@ -54,8 +59,8 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(BUILD, src + ".o")
         _run([hipcc, "-std=c++17", "-O3", "-fPIC", "--offload-arch=" + ARCH,
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-              "-Wno-unused-command-line-argument",
-              "-c", os.path.join(CSRC, src), "-o", obj])
+              "-Wno-unused-command-line-argument"] + HIP_EXTRA.get(src, []) +
+             ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-Wl,-soname,libffip_hip.so", "-o", tmp] + objs)
